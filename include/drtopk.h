@@ -1,0 +1,160 @@
+/*
+ * drtopk.h -- C ABI of the MI355X-native Dr.Top-k library (libdrtopk.so).
+ *
+ * Delegate-centric top-k selection (Dr.Top-k) over one device-resident key
+ * vector: a delegate pass keeps the maximum (or top-beta) of every subrange of
+ * 2^alpha keys, a first top-k over the delegates yields a threshold and the
+ * qualifying subranges, their keys are concatenated, and a second top-k over
+ * the concatenation gives the k-th largest key, the top-k multiset and the
+ * top-k indices.
+ *
+ * Every entry point below replaces one host-side template of the reference,
+ * cited next to it.  The reference passes up to 37 positional parameters and
+ * caller-allocated buffers; here the same parameters (vec, |V|, k, alpha,
+ * beta, digit width, bucket count, filter/beta/shuffle options) travel in
+ * drtopk_params, all scratch lives in one caller-owned workspace whose size
+ * drtopk_workspace_size() returns, and no call allocates device memory.
+ *
+ * Conventions
+ *  - Keys: DRTOPK_U32 (uint32) or DRTOPK_F32 (IEEE float32).  Float keys are
+ *    ordered by their sign-flipped bit pattern: b ^ (b>>31 ? 0xFFFFFFFF :
+ *    0x80000000).  So -0.0 < +0.0, and NaNs order by their bits (positive NaN
+ *    above +inf, negative NaN below -inf).
+ *  - k-th largest = the key at ascending rank n-k (0-based), exactly the
+ *    reference's check vec1[n-k]==TopKElement.
+ *  - Top-k set: every key strictly greater than the k-th largest, plus the
+ *    lowest-index keys equal to it (lowest-index-wins tie rule), written in
+ *    ascending index order: topk_idx[j] < topk_idx[j+1] and
+ *    topk_keys[j] == keys[topk_idx[j]].
+ *  - All calls are stream-ordered and asynchronous unless a host stats pointer
+ *    is passed (then the call synchronises the stream before returning).
+ *  - Errors are returned as drtopk_status; drtopk_last_error() gives a
+ *    thread-local message.  No call exits the process.
+ */
+#ifndef DRTOPK_H
+#define DRTOPK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DRTOPK_VERSION_MAJOR 0
+#define DRTOPK_VERSION_MINOR 1
+
+typedef enum drtopk_status {
+    DRTOPK_OK = 0,
+    DRTOPK_ERR_INVALID_ARG = 1,        /* bad pointer, k==0, k>n, alpha/beta/digit width out of range */
+    DRTOPK_ERR_WORKSPACE_TOO_SMALL = 2,
+    DRTOPK_ERR_UNSUPPORTED = 3,        /* valid but not implemented (e.g. n >= 2^32) */
+    DRTOPK_ERR_HIP = 4,                /* a HIP runtime call failed */
+    DRTOPK_ERR_INTERNAL = 5,           /* a device-side consistency check failed */
+    DRTOPK_ERR_BUDGET = 6              /* serving: request larger than the memory budget */
+} drtopk_status;
+
+typedef enum drtopk_algo {
+    /* Radix select family: sample_radix_select (baseline+filter+beta+shuffle/radixselect.cuh:1783)
+     * and, with alpha == 0, radix_select (radixselect.cuh:1473). */
+    DRTOPK_ALGO_RADIX = 0,
+    /* Bucket select family: sample_bucket_select (bucket_select/bucketselect_combined.cuh:1280)
+     * and, with alpha == 0, bucket_select (bucketselect_combined.cuh:473). */
+    DRTOPK_ALGO_BUCKET = 1,
+    /* Bitonic family: bitonic_firstTopk + bitonic (bitonic/bitonic_com.cuh:610, :669), driven as in
+     * bitonic/sampleBitonic.cu:283-308; alpha == 0 is the plain bitonic top-k (NormalBitonic=1). */
+    DRTOPK_ALGO_BITONIC = 2
+} drtopk_algo;
+
+typedef enum drtopk_dtype {
+    DRTOPK_U32 = 0,
+    DRTOPK_F32 = 1
+} drtopk_dtype;
+
+/* Option bits (reference experiment folders: baseline, baseline+filter, baseline+filter+beta,
+ * baseline+filter+shuffle, baseline+filter+beta+shuffle). */
+#define DRTOPK_FLAG_FILTER   0x1u  /* concatenate only keys >= first-top-k threshold (radixselect.cuh:1575-1639) */
+#define DRTOPK_FLAG_BETA     0x2u  /* alpha<=5: top-beta delegates + lonely rule (radixselect.cuh:472-600) */
+#define DRTOPK_FLAG_SHUFFLE  0x4u  /* alpha<=5: LDS-staged thread-per-subrange sampling (radixselect.cuh:967-1041) */
+#define DRTOPK_FLAG_TIMING   0x100u /* record per-phase device times into drtopk_stats (synchronises) */
+#define DRTOPK_FLAGS_DEFAULT (DRTOPK_FLAG_FILTER | DRTOPK_FLAG_BETA | DRTOPK_FLAG_SHUFFLE)
+
+#define DRTOPK_ALPHA_AUTO (-1)
+
+typedef struct drtopk_params {
+    int32_t alpha;        /* log2 of the subrange size; DRTOPK_ALPHA_AUTO = reference policy; 0 = no delegates */
+    int32_t beta;         /* delegates per subrange on the beta path, 1..4 (reference: argv beta, default 2) */
+    int32_t digit_bits;   /* radix digit width, one of 1,2,4,8,16 (reference NBits, default 8) */
+    int32_t num_buckets;  /* bucket family: buckets per refinement level, 4..4096 (reference NBucket, 256) */
+    uint32_t flags;       /* DRTOPK_FLAG_* */
+    uint32_t reserved[3]; /* must be zero */
+} drtopk_params;
+
+/* Intermediate results, for parity checks and the bench; ordered-key bits throughout. */
+typedef struct drtopk_stats {
+    int32_t algo;
+    int32_t alpha;            /* resolved alpha (0 = no delegates) */
+    int32_t beta;             /* resolved delegates per subrange (1 on the default path) */
+    int32_t path;             /* 0 = plain (no delegates), 1 = default (one max per subrange), 2 = beta path */
+    int32_t digit_bits;
+    int32_t num_buckets;
+    int32_t first_passes;     /* histogram passes of the first top-k (the fused one included) */
+    int32_t second_passes;    /* histogram passes of the second top-k */
+    uint64_t num_subranges;
+    uint64_t num_delegates;
+    uint32_t threshold;       /* first top-k threshold (radix flag / bucket threshold / k-th delegate) */
+    uint32_t kth_bits;        /* ordered-key bits of the k-th largest key */
+    uint64_t selected_subranges;
+    uint64_t lonely_subranges;
+    uint64_t lonely_elements; /* beta delegates per lonely subrange, as the reference counts them */
+    uint64_t concat_size;
+    uint64_t count_greater;   /* keys strictly greater than the k-th largest */
+    uint64_t ties_taken;      /* keys equal to the k-th largest in the top-k set */
+    float phase_ms[5];        /* sample, first top-k, concatenate, second top-k, extract (FLAG_TIMING) */
+    uint32_t reserved[3];
+} drtopk_stats;
+
+/* Fill *p with the reference's final configuration: alpha auto, beta 2, 8-bit digits, 256 buckets,
+ * FILTER|BETA|SHUFFLE (baseline+filter+beta+shuffle/main.cu:83-89, :140-141; Makefile test line). */
+void drtopk_params_default(drtopk_params* p);
+
+/* Bytes of device workspace drtopk_select needs for this problem (an upper bound that does not depend
+ * on the key values). */
+drtopk_status drtopk_workspace_size(drtopk_algo algo, drtopk_dtype dtype, uint64_t n, uint64_t k,
+                                    const drtopk_params* p, size_t* bytes);
+
+/* Top-k of d_keys[0..n).  Replaces sample_radix_select / radix_select, sample_bucket_select /
+ * bucket_select and bitonic_firstTopk + bitonic (see drtopk_algo).  Keys are read only.
+ *   d_kth        device pointer, one key (uint32 or float per dtype), or NULL
+ *   d_topk_keys  device pointer, k keys, or NULL
+ *   d_topk_idx   device pointer, k int64 indices, or NULL
+ *   h_stats      host pointer or NULL; when set the call synchronises the stream and fills it
+ *   stream       a hipStream_t, or NULL for the null stream */
+drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_keys, uint64_t n,
+                            uint64_t k, const drtopk_params* p, void* d_workspace,
+                            size_t workspace_bytes, void* d_kth, void* d_topk_keys,
+                            int64_t* d_topk_idx, drtopk_stats* h_stats, void* stream);
+
+/* Delegate pass alone.  Replaces sampleMax (radixselect.cuh:883-918; bitonic/sampleBitonic.cu:150-185)
+ * for beta == 1 and sampleMax_NoReduction_Beta2/3/4 (radixselect.cuh:967-1202) for beta 2..4.
+ * Subrange s = keys[s*2^alpha, min((s+1)*2^alpha, n)); S = ceil(n / 2^alpha).
+ *   d_vals  [beta*S] delegates of subrange s at [beta*s, beta*s+beta), largest first (a multiset:
+ *           an equal maximum appears twice); a subrange with fewer than beta keys pads with
+ *           ordered key 0 (uint32 0; float bit pattern 0xFFFFFFFF) and position 0xFFFFFFFF.
+ *   d_pos   [beta*S] absolute index of each delegate (ties: lowest index first), or NULL
+ *   flags   DRTOPK_FLAG_SHUFFLE selects the LDS-staged variant; results are identical. */
+drtopk_status drtopk_sample_delegates(drtopk_dtype dtype, const void* d_keys, uint64_t n,
+                                      int32_t alpha, int32_t beta, uint32_t flags, void* d_vals,
+                                      uint32_t* d_pos, void* stream);
+
+/* Thread-local message for the last error on this thread ("" if none). */
+const char* drtopk_last_error(void);
+
+/* "major.minor gfx950" */
+const char* drtopk_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DRTOPK_H */
