@@ -1,0 +1,173 @@
+"""Benchmark: Dr.Top-k on one MI355X at |V| = 2^30 (uint32 uniform, seed 2).
+
+One step = the reference's headline sweep (baseline+filter+beta+shuffle/run.bash:14-22):
+top-k of the same device-resident 2^30-key vector for every k = 2^0 .. 2^24, in the
+reference's final configuration (FILTER|BETA|SHUFFLE, 8-bit digits, beta 2, auto alpha),
+each call returning the k-th key, the top-k keys and their int64 indices.
+value = keys examined per second over the sweep (25 * 2^30 / step time), in Gelem/s.
+
+Printed as one JSON line; also carries the per-k times, the HBM roofline of the
+dominant kernel (the delegate pass) and the CPU baseline timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+METRIC = "top-k Gelem/s + %HBM peak at |V|=2^30; batched req/s + p50/p99 latency"
+
+
+def algorithmic_bytes(st: dict, n: int, k: int) -> int:
+    """4n (the delegate pass reads V once) + second-pass traffic: delegate write, first-top-k
+    passes over delegates, reads of the selected subranges, concatenation write, second-top-k
+    passes over it, and k keys + int64 indices out."""
+    D = st["num_delegates"]
+    C = st["concat_size"]
+    L = 1 << st["alpha"]
+    p1 = max(0, st["first_passes"] - 1) + 1     # unfused digit passes + the range-selection pass
+    p2 = max(0, st["second_passes"] - 1) + 1    # unfused digit passes + the extraction pass
+    if st["path"] == 2:
+        sel_read = st["selected_subranges"] * L * 4
+    else:
+        sel_read = st["selected_subranges"] * L * 4
+    return 4 * n + 4 * D + 4 * D * p1 + sel_read + 8 * C + 4 * C * p2 + 12 * k
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log2n", type=int, default=30)
+    ap.add_argument("--kmax-log2", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU exchange is "
+              "design-only in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import drtopksc_amd as dt
+    from oracle import oracle
+
+    n = 1 << args.log2n
+    ks = [1 << j for j in range(args.kmax_log2 + 1)]
+    t0 = time.time()
+    v = oracle.generate(oracle.UNIFORM_U32, 2, n)
+    print(f"[bench] generated {n} keys in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    dev = torch.from_numpy(v.view(np.int32)).cuda()
+    p = dt.params()
+    ws_bytes = max(dt.workspace_size(n, k, dt.ALGO_RADIX, 0, p) for k in ks)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    kmax = max(ks)
+    kth = torch.empty(1, dtype=dev.dtype, device="cuda")
+    okeys = torch.empty(kmax, dtype=dev.dtype, device="cuda")
+    oidx = torch.empty(kmax, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def call(k, stats=False, timing=False):
+        pp = dt.params(flags=dt.FLAGS_DEFAULT | (dt.FLAG_TIMING if timing else 0))
+        return dt.select(dev, k, p=pp, stats=stats, workspace=ws, out=(kth, okeys[:k], oidx[:k]), stream=stream)
+
+    def sweep():
+        for k in ks:
+            call(k)
+
+    # per-k stats + phase times (outside the timed region) and a correctness spot check
+    per_k = []
+    for k in ks:
+        _, _, _, st = call(k, stats=True, timing=True)
+        for _ in range(2):
+            _, _, _, st = call(k, stats=True, timing=True)
+        per_k.append(st)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        sweep()
+    torch.cuda.synchronize()
+    # per-k wall times with HIP events on the launch stream
+    k_ms = []
+    for k in ks:
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        e0.record(stream)
+        for _ in range(reps):
+            call(k)
+        e1.record(stream)
+        e1.synchronize()
+        k_ms.append(e0.elapsed_time(e1) / reps)
+
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_start = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        sweep()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ms_step = e0.elapsed_time(e1) / args.steps
+    value = len(ks) * n / (ms_step * 1e-3) / 1e9
+
+    # roofline of the dominant kernel: the delegate pass (phase 0), k = 2^10 (alpha 11)
+    rows = []
+    for k, st, ms in zip(ks, per_k, k_ms):
+        B = algorithmic_bytes(st, n, k)
+        rows.append({"k": k, "alpha": st["alpha"], "path": st["path"], "ms": round(ms, 4),
+                     "gelem_s": round(n / (ms * 1e-3) / 1e9, 1),
+                     "hbm_frac": round(B / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                     "hbm_frac_4n": round(4 * n / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                     "concat": st["concat_size"], "phase_ms": [round(x, 4) for x in st["phase_ms"]]})
+    samp = [st["phase_ms"][0] for st in per_k]
+    ref_i = ks.index(1 << 10) if (1 << 10) in ks else 0
+    st_ref = per_k[ref_i]
+    samp_bytes = 4 * n + 4 * st_ref["num_delegates"]
+    samp_ms = samp[ref_i]
+    achieved = samp_bytes / (samp_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "delegate pass (k_sample_waves, alpha=11)", "bytes_per_launch": samp_bytes,
+                "avg_ms": round(samp_ms, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        kc = 1 << 10
+        threads = min(16, os.cpu_count() or 1)
+        t_cpu, K, _, _ = oracle.cpu_baseline(v, kc, threads=threads)
+        kth_gpu = None
+        call(kc)
+        torch.cuda.synchronize()
+        kth_gpu = int(kth.cpu().numpy().view(np.uint32)[0])
+        assert kth_gpu == K, (kth_gpu, K)
+        cpu = {"value": round(n / t_cpu / 1e9, 3), "unit": "Gelem/s", "cores": threads, "kind": "port",
+               "sample": f"full 2^{args.log2n} uniform vector, k=2^10 (nth_element per chunk + merge + sort + "
+                         f"index pass), {t_cpu:.2f}s"}
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Gelem/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"single-vector top-k sweep, |V|=2^{args.log2n} uint32 uniform (mt19937 seed 2), "
+                               f"k=2^0..2^{args.kmax_log2}, keys+int64 indices out",
+                   "n": n, "k_sweep": [ks[0], ks[-1]], "algo": "radix", "flags": "FILTER|BETA|SHUFFLE",
+                   "digit_bits": 8, "beta": 2, "alpha": "auto (reference policy)"},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "min_hbm_frac": min(r["hbm_frac"] for r in rows),
+        "per_k": rows, "wall_s": round(wall, 3),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+"""ctypes binding of libdrtopk.so (include/drtopk.h).
+
+Device memory and streams come from torch (ROCm build); the selection itself runs in the
+HIP kernels of the library.  Mirrors the reference's entry points:
+  select(..., algo=ALGO_RADIX)   sample_radix_select / radix_select
+                                 (baseline+filter+beta+shuffle/radixselect.cuh:1783, :1473)
+  select(..., algo=ALGO_BUCKET)  sample_bucket_select / bucket_select
+                                 (bucket_select/bucketselect_combined.cuh:1280, :473)
+  select(..., algo=ALGO_BITONIC) bitonic_firstTopk + bitonic (bitonic/bitonic_com.cuh:610, :669)
+  sample_delegates(...)          sampleMax / sampleMax_NoReduction_Beta* (radixselect.cuh:883, :967)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(HERE, "lib", "libdrtopk.so")
+
+ALGO_RADIX, ALGO_BUCKET, ALGO_BITONIC = 0, 1, 2
+U32, F32 = 0, 1
+FLAG_FILTER, FLAG_BETA, FLAG_SHUFFLE, FLAG_TIMING = 0x1, 0x2, 0x4, 0x100
+FLAGS_DEFAULT = FLAG_FILTER | FLAG_BETA | FLAG_SHUFFLE
+
+_STATUS = {0: "OK", 1: "INVALID_ARG", 2: "WORKSPACE_TOO_SMALL", 3: "UNSUPPORTED", 4: "HIP", 5: "INTERNAL",
+           6: "BUDGET"}
+
+
+class DrTopKError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"drtopk {_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("alpha", ctypes.c_int32), ("beta", ctypes.c_int32), ("digit_bits", ctypes.c_int32),
+                ("num_buckets", ctypes.c_int32), ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("algo", ctypes.c_int32), ("alpha", ctypes.c_int32), ("beta", ctypes.c_int32),
+                ("path", ctypes.c_int32), ("digit_bits", ctypes.c_int32), ("num_buckets", ctypes.c_int32),
+                ("first_passes", ctypes.c_int32), ("second_passes", ctypes.c_int32),
+                ("num_subranges", ctypes.c_uint64), ("num_delegates", ctypes.c_uint64),
+                ("threshold", ctypes.c_uint32), ("kth_bits", ctypes.c_uint32),
+                ("selected_subranges", ctypes.c_uint64), ("lonely_subranges", ctypes.c_uint64),
+                ("lonely_elements", ctypes.c_uint64), ("concat_size", ctypes.c_uint64),
+                ("count_greater", ctypes.c_uint64), ("ties_taken", ctypes.c_uint64),
+                ("phase_ms", ctypes.c_float * 5), ("reserved", ctypes.c_uint32 * 3)]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name not in ("reserved", "phase_ms")}
+        d["phase_ms"] = list(self.phase_ms)
+        return d
+
+
+def params(alpha=-1, beta=2, digit_bits=8, num_buckets=256, flags=FLAGS_DEFAULT) -> Params:
+    return Params(alpha, beta, digit_bits, num_buckets, flags, (ctypes.c_uint32 * 3)(0, 0, 0))
+
+
+_lib = None
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def lib():
+    """Load libdrtopk.so; raises (no fallback) when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise DrTopKError(4, f"{_LIB_PATH} is missing: run `python -m drtopksc_amd.build`")
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, u64, i32, u32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_uint32
+        L.drtopk_params_default.argtypes = [ctypes.POINTER(Params)]
+        L.drtopk_params_default.restype = None
+        L.drtopk_workspace_size.argtypes = [ctypes.c_int, ctypes.c_int, u64, u64, ctypes.POINTER(Params),
+                                            ctypes.POINTER(ctypes.c_size_t)]
+        L.drtopk_workspace_size.restype = ctypes.c_int
+        L.drtopk_select.argtypes = [ctypes.c_int, ctypes.c_int, vp, u64, u64, ctypes.POINTER(Params), vp,
+                                    ctypes.c_size_t, vp, vp, vp, ctypes.POINTER(Stats), vp]
+        L.drtopk_select.restype = ctypes.c_int
+        L.drtopk_sample_delegates.argtypes = [ctypes.c_int, vp, u64, i32, i32, u32, vp, vp, vp]
+        L.drtopk_sample_delegates.restype = ctypes.c_int
+        L.drtopk_last_error.argtypes = []
+        L.drtopk_last_error.restype = ctypes.c_char_p
+        L.drtopk_version.argtypes = []
+        L.drtopk_version.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise DrTopKError(rc, lib().drtopk_last_error().decode())
+
+
+def _dtype_code(t) -> int:
+    import torch
+    if t.dtype == torch.float32:
+        return F32
+    if t.dtype in (torch.uint32, torch.int32):
+        return U32
+    raise DrTopKError(1, f"keys must be uint32 or float32, got {t.dtype}")
+
+
+def workspace_size(n: int, k: int, algo: int = ALGO_RADIX, dtype: int = U32, p: Params | None = None) -> int:
+    p = p or params()
+    out = ctypes.c_size_t()
+    _check(lib().drtopk_workspace_size(algo, dtype, n, k, ctypes.byref(p), ctypes.byref(out)))
+    return int(out.value)
+
+
+def _stream_ptr(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def select(keys, k: int, algo: int = ALGO_RADIX, p: Params | None = None, want_keys: bool = True,
+           want_idx: bool = True, stats: bool = False, workspace=None, out=None, stream=None):
+    """Top-k of a 1-D device tensor (uint32 / int32 bit patterns, or float32).
+
+    Returns (kth, topk_keys, topk_idx, stats): kth is a 1-element tensor of the key dtype,
+    topk_keys / topk_idx (int64) are in ascending index order (None when not requested),
+    stats a dict when stats=True (the call then synchronises).
+    """
+    import torch
+    if not keys.is_cuda:
+        raise DrTopKError(1, "keys must be a device tensor")
+    keys = keys.contiguous()
+    p = p or params()
+    dt = _dtype_code(keys)
+    n = keys.numel()
+    ws_bytes = workspace_size(n, k, algo, dt, p)
+    if workspace is None or workspace.numel() < ws_bytes:
+        workspace = torch.empty(max(ws_bytes, 256), dtype=torch.uint8, device=keys.device)
+    if out is None:
+        kth = torch.empty(1, dtype=keys.dtype, device=keys.device)
+        tk = torch.empty(k, dtype=keys.dtype, device=keys.device) if want_keys else None
+        ti = torch.empty(k, dtype=torch.int64, device=keys.device) if want_idx else None
+    else:
+        kth, tk, ti = out
+    st = Stats()
+    rc = lib().drtopk_select(algo, dt, ctypes.c_void_p(keys.data_ptr()), n, k, ctypes.byref(p),
+                             ctypes.c_void_p(workspace.data_ptr()), workspace.numel(),
+                             ctypes.c_void_p(kth.data_ptr()),
+                             ctypes.c_void_p(tk.data_ptr()) if tk is not None else None,
+                             ctypes.c_void_p(ti.data_ptr()) if ti is not None else None,
+                             ctypes.byref(st) if stats else None, _stream_ptr(stream))
+    _check(rc)
+    return kth, tk, ti, (st.as_dict() if stats else None)
+
+
+def sample_delegates(keys, alpha: int, beta: int = 1, shuffle: bool = False, want_pos: bool = True, stream=None):
+    """Delegate pass alone: (values [beta*S] in the key dtype, positions [beta*S] uint32 or None)."""
+    import torch
+    keys = keys.contiguous()
+    dt = _dtype_code(keys)
+    n = keys.numel()
+    S = (n + (1 << alpha) - 1) >> alpha
+    vals = torch.empty(beta * S, dtype=keys.dtype, device=keys.device)
+    pos = torch.empty(beta * S, dtype=torch.int32, device=keys.device) if want_pos else None
+    rc = lib().drtopk_sample_delegates(dt, ctypes.c_void_p(keys.data_ptr()), n, alpha, beta,
+                                       FLAG_SHUFFLE if shuffle else 0, ctypes.c_void_p(vals.data_ptr()),
+                                       ctypes.c_void_p(pos.data_ptr()) if pos is not None else None,
+                                       _stream_ptr(stream))
+    _check(rc)
+    return vals, pos

@@ -1,0 +1,428 @@
+// api.hip -- host orchestration and the C ABI (include/drtopk.h).
+//
+// The reference drives each phase from the host with a cudaDeviceSynchronize, a 1 KB
+// device-to-host copy and a host scan per radix digit (radixselect.cuh:1427-1445,
+// :192-220).  Here the whole pipeline is one stream-ordered sequence of kernels: every
+// digit / bucket decision is made by k_select on the device and read by the next kernel
+// from the control block, so a call issues no host synchronisation (unless the caller
+// asks for stats) and can be captured in a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/drtopk.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace drtopk;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+drtopk_status fail(drtopk_status s, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+drtopk_status fail(drtopk_status s, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return s;
+}
+
+int floor_log2(u64 x) { int l = -1; while (x) { x >>= 1; ++l; } return l; }
+int ceil_log2(u64 x) { int l = floor_log2(x); return ((1ull << l) == x) ? l : l + 1; }
+u64 align_up(u64 x, u64 a) { return (x + a - 1) / a * a; }
+u64 cdiv(u64 a, u64 b) { return (a + b - 1) / b; }
+
+constexpr u64 TILE = 4096;       // select.hip tiles
+constexpr u64 BETA_TILE = 1024;  // subranges per beta-path tile
+constexpr u64 MAX_N = (1ull << 31) - 1;
+
+u64 real_delegates(u64 n, int alpha, int beta) {
+    u64 L = 1ull << alpha, S = cdiv(n, L), last = n - (S - 1) * L;
+    return (u64)beta * (S - 1) + std::min<u64>((u64)beta, last);
+}
+
+// Resolved problem + workspace layout.
+struct Plan {
+    drtopk_algo algo;
+    int f32;
+    u64 n, k;
+    int alpha, beta, path, nb, B;
+    u32 flags;
+    u64 L, S, D;
+    u64 cap;          // concatenation capacity
+    int hist_bins;    // histogram buffer bins
+    u64 tiles_a, tiles_b, tiles_c;
+    // byte offsets
+    u64 o_ctrl, o_hist, o_st_a, o_st_b, o_st_c, o_zero_end;
+    u64 o_dvals, o_doff, o_sel, o_ckeys, o_cidx, total;
+};
+
+// Parameter policy: radix alpha = floor(0.5*(log2|V| - log2 k + 3)), beta path iff alpha<=5
+// (baseline+filter+beta+shuffle/main.cu:83-89); bucket: alpha++ when alpha<5
+// (bucket_select/main.cu:66-67); bitonic: user alpha, auto = radix formula in [5,16].
+drtopk_status resolve(drtopk_algo algo, drtopk_dtype dtype, u64 n, u64 k, const drtopk_params* p, Plan* pl) {
+    if (algo < DRTOPK_ALGO_RADIX || algo > DRTOPK_ALGO_BITONIC) return fail(DRTOPK_ERR_INVALID_ARG, "bad algo %d", (int)algo);
+    if (dtype != DRTOPK_U32 && dtype != DRTOPK_F32) return fail(DRTOPK_ERR_INVALID_ARG, "bad dtype %d", (int)dtype);
+    if (!p) return fail(DRTOPK_ERR_INVALID_ARG, "params is NULL");
+    if (p->reserved[0] || p->reserved[1] || p->reserved[2]) return fail(DRTOPK_ERR_INVALID_ARG, "reserved params must be 0");
+    if (n == 0) return fail(DRTOPK_ERR_INVALID_ARG, "n == 0");
+    if (n > MAX_N) return fail(DRTOPK_ERR_UNSUPPORTED, "n = %llu exceeds 2^31-1", (unsigned long long)n);
+    if (k == 0 || k > n) return fail(DRTOPK_ERR_INVALID_ARG, "k = %llu outside [1, n = %llu]", (unsigned long long)k, (unsigned long long)n);
+    const int nb = p->digit_bits;
+    if (nb != 1 && nb != 2 && nb != 4 && nb != 8 && nb != 16) return fail(DRTOPK_ERR_INVALID_ARG, "digit_bits %d not in {1,2,4,8,16}", nb);
+    if (p->num_buckets < 4 || p->num_buckets > 4096) return fail(DRTOPK_ERR_INVALID_ARG, "num_buckets %d not in [4, 4096]", p->num_buckets);
+    if (p->beta < 1 || p->beta > 4) return fail(DRTOPK_ERR_INVALID_ARG, "beta %d not in [1, 4]", p->beta);
+    if (p->alpha < -1 || p->alpha > 30) return fail(DRTOPK_ERR_INVALID_ARG, "alpha %d not in [-1, 30]", p->alpha);
+    if (algo == DRTOPK_ALGO_BITONIC) return fail(DRTOPK_ERR_UNSUPPORTED, "bitonic family not built yet");
+    const int Ln = floor_log2(n);
+    int alpha = p->alpha;
+    const bool autoa = alpha < 0;
+    if (autoa) {
+        alpha = (Ln + 3 - ceil_log2(k)) / 2;
+        if (alpha < 0) alpha = 0;
+        if (algo == DRTOPK_ALGO_BUCKET && alpha < 5) alpha++;
+        if (algo == DRTOPK_ALGO_BITONIC) alpha = std::min(16, std::max(5, alpha));
+        if (alpha > Ln) alpha = Ln;
+    }
+    if (alpha > Ln) return fail(DRTOPK_ERR_INVALID_ARG, "alpha %d larger than log2(n) = %d", alpha, Ln);
+    auto want_beta = [&](int a) {
+        return algo != DRTOPK_ALGO_BITONIC && (p->flags & DRTOPK_FLAG_BETA) && a <= 5 && p->beta >= 2;
+    };
+    auto valid = [&](int a) -> bool {
+        if (a == 0) return true;
+        u64 L = 1ull << a, S = cdiv(n, L);
+        if (algo == DRTOPK_ALGO_BITONIC) return S > k;
+        if (want_beta(a)) return p->beta <= (int)L && real_delegates(n, a, p->beta) >= k;
+        return S >= k;
+    };
+    if (!valid(alpha)) {
+        if (!autoa)
+            return fail(DRTOPK_ERR_INVALID_ARG, "alpha %d leaves fewer than k = %llu delegates", alpha,
+                        (unsigned long long)k);
+        while (alpha > 0 && !valid(alpha)) --alpha;
+    }
+    int beta = 1, path = alpha == 0 ? 0 : 1;
+    if (alpha > 0 && want_beta(alpha)) { beta = p->beta; path = 2; }
+    pl->algo = algo;
+    pl->f32 = dtype == DRTOPK_F32;
+    pl->n = n; pl->k = k;
+    pl->alpha = alpha; pl->beta = beta; pl->path = path; pl->nb = nb; pl->B = p->num_buckets;
+    pl->flags = p->flags;
+    pl->L = 1ull << alpha;
+    pl->S = alpha ? cdiv(n, pl->L) : 0;
+    pl->D = (u64)beta * pl->S;
+    // layout
+    if (path == 0) pl->cap = 0;
+    else if (path == 1) pl->cap = std::min<u64>(n, k * pl->L);
+    else pl->cap = n;
+    pl->hist_bins = std::max<int>(256, std::max<int>(1 << nb, pl->B));
+    pl->tiles_a = path == 2 ? cdiv(pl->S, BETA_TILE) : cdiv(std::max<u64>(pl->S, 1), TILE);
+    pl->tiles_b = path == 1 ? cdiv(std::min<u64>(k * pl->L, pl->S * pl->L), TILE) : 1;
+    pl->tiles_c = cdiv(path == 0 ? n : std::max<u64>(pl->cap, 1), TILE);
+    u64 o = 0;
+    pl->o_ctrl = o; o = align_up(o + sizeof(Ctrl), 256);
+    pl->o_hist = o; o = align_up(o + 4ull * pl->hist_bins, 256);
+    pl->o_st_a = o; o = align_up(o + 8ull * pl->tiles_a, 256);
+    pl->o_st_b = o; o = align_up(o + 8ull * pl->tiles_b, 256);
+    pl->o_st_c = o; o = align_up(o + 8ull * pl->tiles_c, 256);
+    pl->o_zero_end = o;
+    pl->o_dvals = o; o = align_up(o + 4ull * pl->D, 256);
+    u64 offb = path == 2 ? (beta == 2 ? 1 : beta == 3 ? 2 : 4) : 0;
+    pl->o_doff = o; o = align_up(o + offb * pl->S, 256);
+    pl->o_sel = o; o = align_up(o + (path == 1 ? 4ull * k : 0), 256);
+    pl->o_ckeys = o; o = align_up(o + 4ull * pl->cap, 256);
+    pl->o_cidx = o; o = align_up(o + 4ull * pl->cap, 256);
+    pl->total = o;
+    return DRTOPK_OK;
+}
+
+SelState make_sel(int mode, u64 m, u64 k, int nb, int B, int stop_digit, int stop_on_one, int max_passes) {
+    SelState s;
+    std::memset(&s, 0, sizeof(s));
+    s.m = m;
+    s.k = k;
+    if (mode == 0) {
+        s.nb = (u32)nb;
+        s.digit = 32 / nb - 1;
+        s.stop_digit = (u32)stop_digit;
+        if (s.digit < stop_digit) { s.done = 1; s.result = 0; }
+    } else {
+        s.nb = (u32)B;
+        s.lo = 0xFFFFFFFFu;
+        s.hi = 0;
+        s.max_passes = (u32)max_passes;
+    }
+    s.stop_on_one = (u32)stop_on_one;
+    return s;
+}
+
+int bucket_max_levels(int B) {
+    return (int)std::ceil(32.0 / std::log2((double)(B - 1))) + 2;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(DRTOPK_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+void drtopk_params_default(drtopk_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->alpha = DRTOPK_ALPHA_AUTO;
+    p->beta = 2;
+    p->digit_bits = 8;
+    p->num_buckets = 256;
+    p->flags = DRTOPK_FLAGS_DEFAULT;
+}
+
+const char* drtopk_last_error(void) { return g_last_error.c_str(); }
+
+const char* drtopk_version(void) { return "0.1 gfx950"; }
+
+drtopk_status drtopk_workspace_size(drtopk_algo algo, drtopk_dtype dtype, uint64_t n, uint64_t k,
+                                    const drtopk_params* p, size_t* bytes) {
+    if (!bytes) return fail(DRTOPK_ERR_INVALID_ARG, "bytes is NULL");
+    Plan pl;
+    drtopk_status s = resolve(algo, dtype, n, k, p, &pl);
+    if (s != DRTOPK_OK) return s;
+    *bytes = (size_t)pl.total;
+    return DRTOPK_OK;
+}
+
+drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_keys, uint64_t n, uint64_t k,
+                            const drtopk_params* p, void* d_workspace, size_t workspace_bytes, void* d_kth,
+                            void* d_topk_keys, int64_t* d_topk_idx, drtopk_stats* h_stats, void* stream) {
+    g_last_error.clear();
+    Plan pl;
+    drtopk_status s = resolve(algo, dtype, n, k, p, &pl);
+    if (s != DRTOPK_OK) return s;
+    if (!d_keys) return fail(DRTOPK_ERR_INVALID_ARG, "d_keys is NULL");
+    if (((uintptr_t)d_keys & 15) != 0) return fail(DRTOPK_ERR_INVALID_ARG, "d_keys must be 16-byte aligned");
+    if (!d_workspace) return fail(DRTOPK_ERR_INVALID_ARG, "d_workspace is NULL");
+    if (((uintptr_t)d_workspace & 255) != 0) return fail(DRTOPK_ERR_INVALID_ARG, "d_workspace must be 256-byte aligned");
+    if (workspace_bytes < pl.total)
+        return fail(DRTOPK_ERR_WORKSPACE_TOO_SMALL, "workspace %zu bytes < %llu needed", workspace_bytes,
+                    (unsigned long long)pl.total);
+    hipStream_t st = (hipStream_t)stream;
+    char* ws = (char*)d_workspace;
+    Ctrl* ctrl = (Ctrl*)(ws + pl.o_ctrl);
+    u32* hist = (u32*)(ws + pl.o_hist);
+    u64* st_a = (u64*)(ws + pl.o_st_a);
+    u64* st_b = (u64*)(ws + pl.o_st_b);
+    u64* st_c = (u64*)(ws + pl.o_st_c);
+    u32* dvals = (u32*)(ws + pl.o_dvals);
+    void* doff = (void*)(ws + pl.o_doff);
+    u32* sel_ids = (u32*)(ws + pl.o_sel);
+    u32* ckeys = (u32*)(ws + pl.o_ckeys);
+    u32* cidx = (u32*)(ws + pl.o_cidx);
+    const u32* keys = (const u32*)d_keys;
+    const bool want_out = d_topk_keys || d_topk_idx;
+    const bool radix = pl.algo == DRTOPK_ALGO_RADIX;
+    const int mode = radix ? 0 : 1;
+    const int nb = pl.nb, B = pl.B;
+    const int D0 = 32 / nb - 1;
+    const int bins = radix ? (1 << nb) : B;
+    const bool fuse_hist = radix && nb <= 8;
+    const int Lmax = bucket_max_levels(B);
+    const bool timing = (pl.flags & DRTOPK_FLAG_TIMING) && h_stats;
+
+    hipEvent_t ev[6] = {};
+    if (timing)
+        for (int i = 0; i < 6; ++i) HIPCHK(hipEventCreate(&ev[i]));
+    auto mark = [&](int i) -> hipError_t { return timing ? hipEventRecord(ev[i], st) : hipSuccess; };
+
+    // ---- control block
+    Ctrl init;
+    std::memset(&init, 0, sizeof(init));
+    if (pl.path == 0) {
+        init.second = make_sel(mode, pl.n, pl.k, nb, B, 0, radix ? 0 : 1, 1 << 20);
+    } else {
+        if (pl.path == 1)
+            init.first = make_sel(mode, pl.S, pl.k, nb, B, 0, 1, 1 << 20);
+        else
+            init.first = make_sel(mode, pl.D, pl.k, nb, B, 3, 0, 2);
+        init.second = make_sel(mode, 0, pl.k, nb, B, 0, radix ? 0 : 1, 1 << 20);
+    }
+    HIPCHK(launch_init(ctrl, init, (u32*)(ws + pl.o_hist), (pl.o_zero_end - pl.o_hist) / 4, st));
+    HIPCHK(mark(0));
+
+    if (pl.path == 0) {
+        // ---- no delegates: radix_select / bucket_select over V (radixselect.cuh:1473,
+        //      bucketselect_combined.cuh:473)
+        HIPCHK(mark(1));
+        HIPCHK(mark(2));
+        SelState* sec = &ctrl->second;
+        const unsigned g = (unsigned)std::min<u64>(2048, cdiv(pl.n, 4096) + 1);
+        if (radix) {
+            for (int d = 0; d <= D0; ++d) {
+                HIPCHK(launch_hist_raw(0, pl.f32, keys, sec, hist, bins, g, st));
+                HIPCHK(launch_select(0, hist, bins, sec, ctrl, st));
+            }
+        } else {
+            HIPCHK(launch_minmax_raw(pl.f32, keys, sec, ctrl, g, st));
+            HIPCHK(launch_bucket_begin(sec, ctrl, 0, st));
+            for (int l = 0; l < Lmax; ++l) {
+                HIPCHK(launch_hist_raw(1, pl.f32, keys, sec, hist, bins, g, st));
+                HIPCHK(launch_select(1, hist, bins, sec, ctrl, st));
+            }
+            HIPCHK(launch_find(keys, pl.f32 ? 1 : 0, sec, g, st));
+        }
+        HIPCHK(mark(3));
+        if (want_out) {
+            ExtractArgs ea{keys, nullptr, 1, pl.f32, pl.n, pl.k, sec, ctrl, d_topk_keys, d_topk_idx, st_c,
+                           &ctrl->tickets[2]};
+            HIPCHK(launch_extract(ea, st));
+        }
+        HIPCHK(launch_finalize(sec, ctrl, pl.f32, d_kth, pl.k, st));
+        HIPCHK(mark(4));
+    } else {
+        SelState* fst = &ctrl->first;
+        SelState* sec = &ctrl->second;
+        // ---- delegate pass (sampleMax / sampleMax_NoReduction_Beta*, radixselect.cuh:1787-1809)
+        SampleArgs sa;
+        sa.keys = keys; sa.n = pl.n; sa.alpha = pl.alpha; sa.beta = pl.beta; sa.f32 = pl.f32;
+        sa.staged = (pl.flags & DRTOPK_FLAG_SHUFFLE) && pl.alpha <= 5;
+        sa.posmode = (pl.path == 2 && want_out) ? 2 : 0;
+        sa.fuse = radix ? (fuse_hist ? 1 : 0) : 2;
+        sa.dvals = dvals; sa.dpos = doff; sa.ghist = hist;
+        sa.hshift = 32 - nb; sa.bins = bins; sa.ctrl = ctrl;
+        HIPCHK(launch_sample(sa, st));
+        HIPCHK(mark(1));
+        // ---- first top-k over the delegates
+        const u64 Dm = pl.path == 1 ? pl.S : pl.D;
+        const unsigned gd = (unsigned)std::min<u64>(2048, cdiv(Dm, 4096) + 1);
+        if (radix) {
+            // default path: digits until the k-digit bucket holds one delegate
+            // (radix_select_inplace_firsttopk :1667-1724); beta path: digits >= 3 (:1747)
+            const int ndig = pl.path == 1 ? D0 + 1 : std::max(0, D0 - 3 + 1);
+            for (int d = 0; d < ndig; ++d) {
+                if (!(d == 0 && fuse_hist)) HIPCHK(launch_hist(0, dvals, fst, hist, bins, gd, st));
+                HIPCHK(launch_select(0, hist, bins, fst, ctrl, st));
+            }
+        } else {
+            HIPCHK(launch_bucket_begin(fst, ctrl, 1, st));
+            const int lv = pl.path == 1 ? Lmax : 2;
+            for (int l = 0; l < lv; ++l) {
+                HIPCHK(launch_hist(1, dvals, fst, hist, bins, gd, st));
+                HIPCHK(launch_select(1, hist, bins, fst, ctrl, st));
+            }
+            if (pl.path == 1) HIPCHK(launch_find(dvals, 0, fst, gd, st));
+        }
+        HIPCHK(mark(2));
+        // ---- range selection + concatenation
+        u32* h2 = fuse_hist ? hist : nullptr;
+        if (pl.path == 1) {
+            RangeArgs ra{dvals, pl.S, pl.k, radix ? 0 : 1, fst, ctrl, sel_ids, st_a, &ctrl->tickets[0]};
+            HIPCHK(launch_range_default(ra, st));
+            GatherArgs ga{keys, pl.n, pl.alpha, pl.f32, (pl.flags & DRTOPK_FLAG_FILTER) ? 1 : 0, sel_ids, pl.k,
+                          ctrl, ckeys, want_out ? cidx : nullptr, pl.cap, st_b, &ctrl->tickets[1], h2, 32 - nb,
+                          bins};
+            HIPCHK(launch_gather_default(ga, st));
+        } else {
+            BetaGatherArgs ba{keys, pl.n, pl.alpha, pl.beta, pl.f32, (pl.flags & DRTOPK_FLAG_FILTER) ? 1 : 0,
+                              dvals, doff, ctrl, ckeys, want_out ? cidx : nullptr, pl.cap, st_a,
+                              &ctrl->tickets[0], h2, 32 - nb, bins};
+            HIPCHK(launch_range_gather_beta(ba, st));
+        }
+        HIPCHK(mark(3));
+        // ---- second top-k over the concatenation (radix_select_inplace :1891 / bucket_select :1355)
+        const unsigned gc = (unsigned)std::min<u64>(2048, cdiv(pl.cap, 4096) + 1);
+        if (radix) {
+            for (int d = 0; d <= D0; ++d) {
+                if (!(d == 0 && fuse_hist)) HIPCHK(launch_hist(0, ckeys, sec, hist, bins, gc, st));
+                HIPCHK(launch_select(0, hist, bins, sec, ctrl, st));
+            }
+        } else {
+            HIPCHK(launch_minmax(ckeys, sec, ctrl, gc, st));
+            HIPCHK(launch_bucket_begin(sec, ctrl, 0, st));
+            for (int l = 0; l < Lmax; ++l) {
+                HIPCHK(launch_hist(1, ckeys, sec, hist, bins, gc, st));
+                HIPCHK(launch_select(1, hist, bins, sec, ctrl, st));
+            }
+            HIPCHK(launch_find(ckeys, 0, sec, gc, st));
+        }
+        HIPCHK(mark(4));
+        if (want_out) {
+            ExtractArgs ea{ckeys, cidx, 0, pl.f32, pl.cap, pl.k, sec, ctrl, d_topk_keys, d_topk_idx, st_c,
+                           &ctrl->tickets[2]};
+            HIPCHK(launch_extract(ea, st));
+        }
+        HIPCHK(launch_finalize(sec, ctrl, pl.f32, d_kth, pl.k, st));
+    }
+    HIPCHK(mark(5));
+
+    if (h_stats) {
+        Ctrl hc;
+        HIPCHK(hipMemcpyAsync(&hc, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        drtopk_stats o;
+        std::memset(&o, 0, sizeof(o));
+        o.algo = (int)pl.algo; o.alpha = pl.alpha; o.beta = pl.beta; o.path = pl.path;
+        o.digit_bits = nb; o.num_buckets = B;
+        o.first_passes = pl.path == 0 ? 0 : (int)hc.first.passes;
+        o.second_passes = (int)hc.second.passes;
+        o.num_subranges = pl.S; o.num_delegates = pl.D;
+        o.threshold = pl.path == 0 ? 0 : hc.first.result;
+        o.kth_bits = hc.kth;
+        o.selected_subranges = hc.selected;
+        o.lonely_subranges = hc.lonely_sub;
+        o.lonely_elements = hc.lonely_el;
+        o.concat_size = pl.path == 0 ? pl.n : hc.concat_size;
+        o.count_greater = hc.count_gt;
+        o.ties_taken = hc.ties;
+        if (timing) {
+            // phases: sample [0,1], first [1,2], concatenate [2,3], second [3,4], extract [4,5]
+            for (int i = 0; i < 5; ++i) {
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+                o.phase_ms[i] = ms;
+            }
+            for (int i = 0; i < 6; ++i) hipEventDestroy(ev[i]);
+        }
+        *h_stats = o;
+        if (hc.error)
+            return fail(DRTOPK_ERR_INTERNAL, "device-side check failed (code 0x%x)", hc.error);
+    }
+    return DRTOPK_OK;
+}
+
+drtopk_status drtopk_sample_delegates(drtopk_dtype dtype, const void* d_keys, uint64_t n, int32_t alpha,
+                                      int32_t beta, uint32_t flags, void* d_vals, uint32_t* d_pos, void* stream) {
+    g_last_error.clear();
+    if (dtype != DRTOPK_U32 && dtype != DRTOPK_F32) return fail(DRTOPK_ERR_INVALID_ARG, "bad dtype");
+    if (!d_keys || !d_vals) return fail(DRTOPK_ERR_INVALID_ARG, "NULL pointer");
+    if (((uintptr_t)d_keys & 15) != 0) return fail(DRTOPK_ERR_INVALID_ARG, "d_keys must be 16-byte aligned");
+    if (n == 0) return fail(DRTOPK_ERR_INVALID_ARG, "n == 0");
+    if (n > MAX_N) return fail(DRTOPK_ERR_UNSUPPORTED, "n exceeds 2^31-1");
+    if (alpha < 1 || alpha > 30 || (1ull << alpha) > std::max<u64>(n, 1)) return fail(DRTOPK_ERR_INVALID_ARG, "alpha %d out of range", alpha);
+    if (beta < 1 || beta > 4 || beta > (1 << alpha)) return fail(DRTOPK_ERR_INVALID_ARG, "beta %d out of range", beta);
+    hipStream_t st = (hipStream_t)stream;
+    SampleArgs sa;
+    sa.keys = (const u32*)d_keys; sa.n = n; sa.alpha = alpha; sa.beta = beta; sa.f32 = dtype == DRTOPK_F32;
+    sa.staged = (flags & DRTOPK_FLAG_SHUFFLE) && alpha <= 5;
+    sa.posmode = d_pos ? 1 : 0;
+    sa.fuse = 0;
+    sa.dvals = (u32*)d_vals; sa.dpos = d_pos; sa.ghist = nullptr; sa.hshift = 0; sa.bins = 0; sa.ctrl = nullptr;
+    HIPCHK(launch_sample(sa, st));
+    if (sa.f32) {
+        const u64 S = cdiv(n, 1ull << alpha);
+        HIPCHK(launch_to_user((const u32*)d_vals, S * (u64)beta, 1, d_vals, st));
+    }
+    return DRTOPK_OK;
+}
+
+}  // extern "C"

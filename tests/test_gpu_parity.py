@@ -1,0 +1,191 @@
+"""GPU parity: the HIP library against the CPU oracle, bit for bit.
+
+Everything goes through the C ABI (libdrtopk.so via drtopksc_amd).  Each case prints its
+name before touching the GPU so a failing run names it.  Outputs compared:
+k-th largest, top-k keys and int64 indices (ascending index order, lowest index wins
+ties), and the intermediate results the reference computes (threshold, selected /
+lonely subranges, concatenation size, histogram passes).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def dt():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import drtopksc_amd as dt
+    dt.lib()
+    return dt
+
+
+def to_dev(v_u32):
+    return torch.from_numpy(np.ascontiguousarray(v_u32).view(np.int32)).cuda()
+
+
+def to_dev_f32(bits_u32):
+    return torch.from_numpy(np.ascontiguousarray(bits_u32).view(np.float32)).cuda()
+
+
+def run_gpu(dt, dev_keys, k, algo, p):
+    kth, tk, ti, st = dt.select(dev_keys, k, algo=algo, p=p, stats=True)
+    torch.cuda.synchronize()
+    return kth.cpu().numpy(), tk.cpu().numpy(), ti.cpu().numpy(), st
+
+
+STAT_KEYS = ["alpha", "beta", "path", "threshold", "kth_bits", "selected_subranges", "lonely_subranges",
+             "lonely_elements", "concat_size", "count_greater", "ties_taken", "first_passes",
+             "num_subranges", "num_delegates"]
+
+
+def compare(orc, dt, name, okeys, dev, k, algo, flags, alpha=-1, beta=2, nb=8, B=256, f32=False):
+    print(f"[case] {name} algo={algo} k={k} alpha={alpha} beta={beta} nb={nb} B={B} flags={flags} "
+          f"n={okeys.size} f32={f32}", flush=True)
+    po = orc.params(alpha=alpha, beta=beta, digit_bits=nb, num_buckets=B, flags=flags)
+    if orc.resolve(algo, okeys.size, k, po) is None:
+        return False
+    K, keys, idx, ost = orc.drtopk(algo, okeys, k, po)
+    pg = dt.params(alpha=alpha, beta=beta, digit_bits=nb, num_buckets=B, flags=flags)
+    gk, gkeys, gidx, gst = run_gpu(dt, dev, k, algo, pg)
+    gk_bits = gk.view(np.uint32)[0]
+    gkeys_bits = gkeys.view(np.uint32)
+    if f32:
+        gk_bits = orc.f32_to_ordered(np.array([gk_bits], np.uint32))[0]
+        gkeys_bits = orc.f32_to_ordered(gkeys_bits)
+    assert int(gk_bits) == K, (name, "kth", int(gk_bits), K)
+    assert np.array_equal(gidx, idx), (name, "idx")
+    assert np.array_equal(gkeys_bits, keys), (name, "keys")
+    for s in STAT_KEYS:
+        assert gst[s] == ost[s], (name, s, gst[s], ost[s])
+    if algo == 0 or ost["path"] != 0:
+        assert gst["second_passes"] == ost["second_passes"] or (ost["path"] == 0 and okeys.size > (1 << 22)), \
+            (name, "second_passes", gst["second_passes"], ost["second_passes"])
+    return True
+
+
+# ------------------------------------------------------------------ delegate pass
+@pytest.mark.parametrize("alpha", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 16])
+@pytest.mark.parametrize("beta", [1, 2, 3, 4])
+def test_sample_delegates(orc, dt, alpha, beta):
+    if beta > (1 << alpha):
+        pytest.skip("beta > subrange size")
+    for n in [(1 << 18), (1 << 18) + 5, 1000]:
+        if (1 << alpha) > n:
+            continue
+        for dist in (0, 2, 3):
+            v = orc.generate(dist, 100 + alpha * 4 + beta, n)
+            ok = orc.f32_to_ordered(v) if dist == 3 else v
+            want_vals, want_pos = orc.sample(ok, alpha, beta)
+            dev = to_dev_f32(v) if dist == 3 else to_dev(v)
+            for shuffle in (False, True):
+                print(f"[case] sample alpha={alpha} beta={beta} n={n} dist={dist} shuffle={shuffle}", flush=True)
+                vals, pos = dt.sample_delegates(dev, alpha, beta, shuffle=shuffle)
+                torch.cuda.synchronize()
+                gv = vals.cpu().numpy().view(np.uint32)
+                if dist == 3:
+                    gv = orc.f32_to_ordered(gv)
+                assert np.array_equal(gv, want_vals)
+                assert np.array_equal(pos.cpu().numpy().view(np.uint32), want_pos)
+
+
+# ------------------------------------------------------------------ full pipeline
+def datasets(orc):
+    rng = np.random.default_rng(7)
+    yield "uniform", orc.generate(0, 2, 1 << 20), False
+    yield "normal", orc.generate(1, 3, 1 << 20), False
+    yield "lognormal", orc.generate(2, 4, 1 << 20), False
+    yield "ties8", rng.integers(0, 8, 300001, dtype=np.uint64).astype(np.uint32), False
+    yield "allequal", np.full(70000, 123456, np.uint32), False
+    yield "ragged", orc.generate(0, 9, 1000003), False
+    yield "f32normal", orc.generate(3, 5, 1 << 20), True
+    yield "f32uniform", orc.generate(4, 6, (1 << 19) + 77), True
+
+
+@pytest.mark.parametrize("algo", [0, 1])
+def test_pipeline_default_config(orc, dt, algo):
+    """The reference's final configuration (FILTER|BETA|SHUFFLE, 8-bit digits, beta 2, auto alpha)."""
+    for name, v, f32 in datasets(orc):
+        ok = orc.f32_to_ordered(v) if f32 else v
+        dev = to_dev_f32(v) if f32 else to_dev(v)
+        n = v.size
+        for k in [1, 2, 7, 128, 1000, 1 << 12, 1 << 15, n // 16, n // 3, n]:
+            compare(orc, dt, name, ok, dev, k, algo, orc.FLAGS_DEFAULT, f32=f32)
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3, 5, 6, 7])
+@pytest.mark.parametrize("nb", [4, 8, 16])
+def test_pipeline_radix_options(orc, dt, flags, nb):
+    """Option surface: filter / beta / shuffle subsets and digit widths, explicit alphas."""
+    rng = np.random.default_rng(flags * 10 + nb)
+    v = orc.generate(0, 11, 1 << 18)
+    t = rng.integers(0, 50, 1 << 17, dtype=np.uint64).astype(np.uint32)
+    for name, arr in (("uniform", v), ("ties50", t)):
+        dev = to_dev(arr)
+        for k, alpha, beta in [(1, -1, 2), (64, 6, 2), (1000, 4, 2), (5000, 3, 3), (3000, 2, 4), (17, 9, 2),
+                               (arr.size // 4, 1, 2), (300, 5, 3)]:
+            compare(orc, dt, name, arr, dev, k, 0, flags, alpha=alpha, beta=beta, nb=nb)
+
+
+@pytest.mark.parametrize("B", [4, 16, 256, 4096])
+def test_pipeline_bucket_options(orc, dt, B):
+    v = orc.generate(1, 12, 1 << 18)
+    dev = to_dev(v)
+    for flags in (7, 5, 1):
+        for k, alpha in [(1, -1), (100, 7), (2000, 4), (9000, 3), (50000, 2)]:
+            compare(orc, dt, "normal", v, dev, k, 1, flags, alpha=alpha, B=B)
+
+
+def test_plain_paths(orc, dt):
+    """alpha == 0: radix_select / bucket_select over V (radixselect.cuh:1473, bucketselect_combined.cuh:473)."""
+    for name, v, f32 in datasets(orc):
+        ok = orc.f32_to_ordered(v) if f32 else v
+        dev = to_dev_f32(v) if f32 else to_dev(v)
+        for algo in (0, 1):
+            for k in (1, 33, v.size // 2, v.size):
+                compare(orc, dt, name, ok, dev, k, algo, orc.FLAGS_DEFAULT, alpha=0, f32=f32)
+
+
+def test_kth_only_and_keys_only(orc, dt):
+    v = orc.generate(2, 4, 1 << 20)
+    dev = to_dev(v)
+    for algo in (0, 1):
+        for k in (1, 999, 1 << 16):
+            K, keys, idx = orc.topk_reference(v, k)
+            kth, tk, ti, _ = dt.select(dev, k, algo=algo, want_keys=False, want_idx=False)
+            assert int(kth.cpu().numpy().view(np.uint32)[0]) == K
+            kth, tk, ti, _ = dt.select(dev, k, algo=algo, want_keys=True, want_idx=False)
+            assert np.array_equal(tk.cpu().numpy().view(np.uint32), keys)
+
+
+def test_repeat_calls_share_workspace(orc, dt):
+    """A workspace is reusable: every call re-initialises it on the device."""
+    v = orc.generate(0, 21, 1 << 20)
+    dev = to_dev(v)
+    ws = torch.empty(dt.workspace_size(v.size, 1 << 16), dtype=torch.uint8, device="cuda")
+    for k in (1 << 16, 5, 1 << 16, 12345):
+        K, keys, idx = orc.topk_reference(v, k)
+        kth, tk, ti, _ = dt.select(dev, k, workspace=ws if k <= (1 << 16) else None)
+        torch.cuda.synchronize()
+        assert int(kth.cpu().numpy().view(np.uint32)[0]) == K
+        assert np.array_equal(ti.cpu().numpy(), idx)
+
+
+def test_full_size_uniform(orc, dt):
+    """|V| = 2^30 uint32 uniform (seed 2): k-th against the OpenMP nth_element oracle, plus properties."""
+    n = 1 << 30
+    v = orc.generate(0, 2, n)
+    dev = to_dev(v)
+    for k in (1, 1 << 10, 1 << 20, 1 << 24):
+        print(f"[case] full-size uniform k={k}", flush=True)
+        kth, tk, ti, st = dt.select(dev, k, stats=True)
+        torch.cuda.synchronize()
+        _, K, keys, idx = orc.cpu_baseline(v, k, threads=0)
+        assert int(kth.cpu().numpy().view(np.uint32)[0]) == K
+        assert np.array_equal(ti.cpu().numpy(), idx)
+        assert np.array_equal(tk.cpu().numpy().view(np.uint32), keys)
+    del dev
+    torch.cuda.empty_cache()
