@@ -17,6 +17,9 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libdrtopk.so")
+CLI_SRC = os.path.join(HERE, "cli", "topk_main.cpp")
+CLI = os.path.join(LIB_DIR, "drtopk_topk")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 ARCH = os.environ.get("DRTOPK_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -53,8 +56,15 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # command-line driver mirroring the reference's main.cu, linked against the library
+    cmd = [HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", CLI_SRC, f"-L{LIB_DIR}", "-ldrtopk",
+           "-Wl,-rpath,$ORIGIN", "-o", CLI + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"cli build failed:\n{r.stderr}")
+    os.replace(CLI + ".tmp", CLI)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CLI}")
     return LIB
 
 

@@ -1,0 +1,100 @@
+// topk_main.cpp -- command-line driver mirroring the reference's main.cu
+// (baseline+filter+beta+shuffle/main.cu:56-323; bucket_select/main.cu; bitonic/sampleBitonic.cu):
+//
+//   drtopk_topk <log2|V|> <k> <NBitsPerDigit> <beta> [--algo radix|bucket|bitonic] [--alpha A]
+//               [--flags F] [--seed S] [--check] [--buckets B]
+//
+// Keys are uint32 drawn from std::mt19937(seed) (the reference seeds from std::random_device,
+// main.cu:75-76).  alpha defaults to the reference policy (main.cu:83).  --check sorts a host
+// copy and compares vec1[n-k] with the result, as the reference's Enabletest block does
+// (main.cu:281-296); the exit status is 0 on success, -1 (255) on a failed check or a bad argument.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "drtopk.hpp"
+
+int main(int argc, char** argv) {
+    if (argc < 5) {
+        std::printf("./exe num_element(log2) k NBitsPerDigit beta [--algo radix|bucket|bitonic] [--alpha A] "
+                    "[--flags F] [--seed S] [--check] [--buckets B]\n");
+        return -1;
+    }
+    const int num_pow = std::atoi(argv[1]);
+    const long long k = std::atoll(argv[2]);
+    const int NBits = std::atoi(argv[3]);
+    const int beta = std::atoi(argv[4]);
+    std::string algo = "radix";
+    int alpha = -1, buckets = 256;
+    unsigned flags = DRTOPK_FLAGS_DEFAULT, seed = 1;
+    bool check = false;
+    for (int i = 5; i < argc; ++i) {
+        std::string a = argv[i];
+        if (a == "--algo" && i + 1 < argc) algo = argv[++i];
+        else if (a == "--alpha" && i + 1 < argc) alpha = std::atoi(argv[++i]);
+        else if (a == "--flags" && i + 1 < argc) flags = (unsigned)std::strtoul(argv[++i], nullptr, 0);
+        else if (a == "--seed" && i + 1 < argc) seed = (unsigned)std::strtoul(argv[++i], nullptr, 0);
+        else if (a == "--buckets" && i + 1 < argc) buckets = std::atoi(argv[++i]);
+        else if (a == "--check") check = true;
+        else { std::printf("wrong input: %s\n", a.c_str()); return -1; }
+    }
+    if (num_pow < 1 || num_pow > 30 || k < 1 || k > (1ll << num_pow)) { std::printf("wrong input\n"); return -1; }
+    const unsigned long long n = 1ull << num_pow;
+    std::printf("num_element: %llu\n", n);
+    std::vector<unsigned> vec(n);
+    std::mt19937 gen(seed);
+    for (auto& v : vec) v = gen();
+    unsigned* vec_d = nullptr;
+    if (hipMalloc(&vec_d, n * 4) != hipSuccess) { std::printf("hipMalloc failed\n"); return -1; }
+    if (hipMemcpy(vec_d, vec.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess) { std::printf("H2D failed\n"); return -1; }
+
+    drtopk_params p;
+    drtopk_params_default(&p);
+    p.alpha = alpha;
+    p.beta = beta;
+    p.digit_bits = NBits;
+    p.num_buckets = buckets;
+    p.flags = flags | DRTOPK_FLAG_TIMING;
+    const drtopk_algo a = algo == "bucket" ? DRTOPK_ALGO_BUCKET : algo == "bitonic" ? DRTOPK_ALGO_BITONIC
+                                                                                     : DRTOPK_ALGO_RADIX;
+    size_t bytes = 0;
+    drtopk_status s = drtopk_workspace_size(a, DRTOPK_U32, n, (uint64_t)k, &p, &bytes);
+    if (s != DRTOPK_OK) { std::printf("error: %s\n", drtopk_last_error()); return -1; }
+    void* ws = nullptr;
+    unsigned* kth_d = nullptr;
+    if (hipMalloc(&ws, bytes) != hipSuccess || hipMalloc(&kth_d, 4) != hipSuccess) { std::printf("hipMalloc failed\n"); return -1; }
+    drtopk_stats st;
+    for (int rep = 0; rep < 2; ++rep) {  // first call warms up the code objects
+        s = drtopk_select(a, DRTOPK_U32, vec_d, n, (uint64_t)k, &p, ws, bytes, kth_d, nullptr, nullptr, &st, nullptr);
+        if (s != DRTOPK_OK) { std::printf("error: %s\n", drtopk_last_error()); return -1; }
+    }
+    unsigned TopKElement = 0;
+    if (hipMemcpy(&TopKElement, kth_d, 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    std::printf("Calculated alpha: %d  beta path: %d\n", st.alpha, st.path == 2);
+    std::printf("The kth element from top is:%u\n", TopKElement);
+    std::printf("Sampling Time:%g ms\n", st.phase_ms[0]);
+    std::printf("Time for First TopK:%g ms\n", st.phase_ms[1]);
+    std::printf("Time for Concatenation:%g ms\n", st.phase_ms[2]);
+    std::printf("Time for Second TopK:%g ms\n", st.phase_ms[3] + st.phase_ms[4]);
+    std::printf("New Concatenated Size: %llu\n", (unsigned long long)st.concat_size);
+    double tot = 0;
+    for (float x : st.phase_ms) tot += x;
+    std::printf("Total Time:%g ms\n", tot);
+    int rc = 0;
+    if (check) {
+        std::nth_element(vec.begin(), vec.begin() + (n - k), vec.end());
+        if (vec[n - k] == TopKElement) std::printf("Success!\n");
+        else { std::printf("Not Success!\n"); rc = -1; }
+        std::printf("Required value%u\n", vec[n - k]);
+    }
+    (void)hipFree(ws);
+    (void)hipFree(kth_d);
+    (void)hipFree(vec_d);
+    return rc;
+}

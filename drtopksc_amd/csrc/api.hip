@@ -41,8 +41,6 @@ int ceil_log2(u64 x) { int l = floor_log2(x); return ((1ull << l) == x) ? l : l 
 u64 align_up(u64 x, u64 a) { return (x + a - 1) / a * a; }
 u64 cdiv(u64 a, u64 b) { return (a + b - 1) / b; }
 
-constexpr u64 TILE = 4096;       // select.hip tiles
-constexpr u64 BETA_TILE = 1024;  // subranges per beta-path tile
 constexpr u64 MAX_N = (1ull << 31) - 1;
 
 u64 real_delegates(u64 n, int alpha, int beta) {
@@ -58,12 +56,14 @@ struct Plan {
     int alpha, beta, path, nb, B;
     u32 flags;
     u64 L, S, D;
-    u64 cap;          // concatenation capacity
-    int hist_bins;    // histogram buffer bins
-    u64 tiles_a, tiles_b, tiles_c;
+    u64 segcap, nseg;  // segmented concatenation (path 1: gather tiles, path 2: beta tiles)
+    u64 ext_nseg;      // extraction segments
+    u64 range_nt;      // range-selection tiles
+    int hist_bins;     // histogram buffer words
     // byte offsets
-    u64 o_ctrl, o_hist, o_st_a, o_st_b, o_st_c, o_zero_end;
-    u64 o_dvals, o_doff, o_sel, o_ckeys, o_cidx, total;
+    u64 o_ctrl, o_hist, o_zero_end;
+    u64 o_rcnt, o_roff, o_ecnt, o_eoff, o_scnt;
+    u64 o_dvals, o_doff, o_sel, o_skeys, o_sidx, total;
 };
 
 // Parameter policy: radix alpha = floor(0.5*(log2|V| - log2 k + 3)), beta path iff alpha<=5
@@ -121,26 +121,33 @@ drtopk_status resolve(drtopk_algo algo, drtopk_dtype dtype, u64 n, u64 k, const 
     pl->S = alpha ? cdiv(n, pl->L) : 0;
     pl->D = (u64)beta * pl->S;
     // layout
-    if (path == 0) pl->cap = 0;
-    else if (path == 1) pl->cap = std::min<u64>(n, k * pl->L);
-    else pl->cap = n;
-    pl->hist_bins = std::max<int>(256, std::max<int>(1 << nb, pl->B));
-    pl->tiles_a = path == 2 ? cdiv(pl->S, BETA_TILE) : cdiv(std::max<u64>(pl->S, 1), TILE);
-    pl->tiles_b = path == 1 ? cdiv(std::min<u64>(k * pl->L, pl->S * pl->L), TILE) : 1;
-    pl->tiles_c = cdiv(path == 0 ? n : std::max<u64>(pl->cap, 1), TILE);
+    pl->segcap = 0;
+    pl->nseg = 0;
+    if (path == 1) {
+        pl->segcap = gather_segcap(k * pl->L);
+        pl->nseg = gather_nseg(k * pl->L);
+    } else if (path == 2) {
+        pl->segcap = beta_segcap(pl->S, alpha);
+        pl->nseg = beta_nseg(pl->S);
+    }
+    pl->ext_nseg = path == 0 ? cdiv(n, DIRECT_TILE) : pl->nseg;
+    pl->range_nt = path == 1 ? range_tiles(pl->S) : 0;
+    pl->hist_bins = std::max<int>(HIST_REPS * 256, std::max<int>(1 << nb, pl->B));
     u64 o = 0;
     pl->o_ctrl = o; o = align_up(o + sizeof(Ctrl), 256);
     pl->o_hist = o; o = align_up(o + 4ull * pl->hist_bins, 256);
-    pl->o_st_a = o; o = align_up(o + 8ull * pl->tiles_a, 256);
-    pl->o_st_b = o; o = align_up(o + 8ull * pl->tiles_b, 256);
-    pl->o_st_c = o; o = align_up(o + 8ull * pl->tiles_c, 256);
     pl->o_zero_end = o;
+    pl->o_rcnt = o; o = align_up(o + 8ull * pl->range_nt, 256);
+    pl->o_roff = o; o = align_up(o + 8ull * pl->range_nt, 256);
+    pl->o_ecnt = o; o = align_up(o + 8ull * pl->ext_nseg, 256);
+    pl->o_eoff = o; o = align_up(o + 8ull * pl->ext_nseg, 256);
+    pl->o_scnt = o; o = align_up(o + 4ull * pl->nseg, 256);
     pl->o_dvals = o; o = align_up(o + 4ull * pl->D, 256);
-    u64 offb = path == 2 ? (beta == 2 ? 1 : beta == 3 ? 2 : 4) : 0;
+    const u64 offb = path == 2 ? (beta == 2 ? 1 : beta == 3 ? 2 : 4) : 0;
     pl->o_doff = o; o = align_up(o + offb * pl->S, 256);
     pl->o_sel = o; o = align_up(o + (path == 1 ? 4ull * k : 0), 256);
-    pl->o_ckeys = o; o = align_up(o + 4ull * pl->cap, 256);
-    pl->o_cidx = o; o = align_up(o + 4ull * pl->cap, 256);
+    pl->o_skeys = o; o = align_up(o + 4ull * pl->segcap * pl->nseg, 256);
+    pl->o_sidx = o; o = align_up(o + 4ull * pl->segcap * pl->nseg, 256);
     pl->total = o;
     return DRTOPK_OK;
 }
@@ -221,14 +228,16 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
     char* ws = (char*)d_workspace;
     Ctrl* ctrl = (Ctrl*)(ws + pl.o_ctrl);
     u32* hist = (u32*)(ws + pl.o_hist);
-    u64* st_a = (u64*)(ws + pl.o_st_a);
-    u64* st_b = (u64*)(ws + pl.o_st_b);
-    u64* st_c = (u64*)(ws + pl.o_st_c);
+    u64* rcnt = (u64*)(ws + pl.o_rcnt);
+    u64* roff = (u64*)(ws + pl.o_roff);
+    u64* ecnt = (u64*)(ws + pl.o_ecnt);
+    u64* eoff = (u64*)(ws + pl.o_eoff);
+    u32* scnt = (u32*)(ws + pl.o_scnt);
     u32* dvals = (u32*)(ws + pl.o_dvals);
     void* doff = (void*)(ws + pl.o_doff);
     u32* sel_ids = (u32*)(ws + pl.o_sel);
-    u32* ckeys = (u32*)(ws + pl.o_ckeys);
-    u32* cidx = (u32*)(ws + pl.o_cidx);
+    u32* skeys = (u32*)(ws + pl.o_skeys);
+    u32* sidx = (u32*)(ws + pl.o_sidx);
     const u32* keys = (const u32*)d_keys;
     const bool want_out = d_topk_keys || d_topk_idx;
     const bool radix = pl.algo == DRTOPK_ALGO_RADIX;
@@ -283,8 +292,8 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
         }
         HIPCHK(mark(3));
         if (want_out) {
-            ExtractArgs ea{keys, nullptr, 1, pl.f32, pl.n, pl.k, sec, ctrl, d_topk_keys, d_topk_idx, st_c,
-                           &ctrl->tickets[2]};
+            ExtractArgs ea{Seg{keys, nullptr, nullptr, DIRECT_TILE, pl.ext_nseg, pl.n}, 1, pl.f32, pl.k, sec, ctrl,
+                           d_topk_keys, d_topk_idx, ecnt, eoff};
             HIPCHK(launch_extract(ea, st));
         }
         HIPCHK(launch_finalize(sec, ctrl, pl.f32, d_kth, pl.k, st));
@@ -323,42 +332,42 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
             if (pl.path == 1) HIPCHK(launch_find(dvals, 0, fst, gd, st));
         }
         HIPCHK(mark(2));
-        // ---- range selection + concatenation
+        // ---- range selection + concatenation (segmented, one segment per tile)
         u32* h2 = fuse_hist ? hist : nullptr;
+        u32* sidx_or_null = want_out ? sidx : nullptr;
         if (pl.path == 1) {
-            RangeArgs ra{dvals, pl.S, pl.k, radix ? 0 : 1, fst, ctrl, sel_ids, st_a, &ctrl->tickets[0]};
-            HIPCHK(launch_range_default(ra, st));
+            RangeArgs ra{dvals, pl.S, pl.k, radix ? 0 : 1, fst, ctrl, sel_ids, rcnt, roff};
+            HIPCHK(launch_range(ra, st));
             GatherArgs ga{keys, pl.n, pl.alpha, pl.f32, (pl.flags & DRTOPK_FLAG_FILTER) ? 1 : 0, sel_ids, pl.k,
-                          ctrl, ckeys, want_out ? cidx : nullptr, pl.cap, st_b, &ctrl->tickets[1], h2, 32 - nb,
-                          bins};
-            HIPCHK(launch_gather_default(ga, st));
+                          ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins};
+            HIPCHK(launch_gather(ga, st));
         } else {
             BetaGatherArgs ba{keys, pl.n, pl.alpha, pl.beta, pl.f32, (pl.flags & DRTOPK_FLAG_FILTER) ? 1 : 0,
-                              dvals, doff, ctrl, ckeys, want_out ? cidx : nullptr, pl.cap, st_a,
-                              &ctrl->tickets[0], h2, 32 - nb, bins};
-            HIPCHK(launch_range_gather_beta(ba, st));
+                              dvals, doff, ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins};
+            HIPCHK(launch_beta_gather(ba, st));
         }
+        HIPCHK(launch_seg_total(scnt, pl.nseg, ctrl, st));
         HIPCHK(mark(3));
         // ---- second top-k over the concatenation (radix_select_inplace :1891 / bucket_select :1355)
-        const unsigned gc = (unsigned)std::min<u64>(2048, cdiv(pl.cap, 4096) + 1);
+        const Seg cseg{skeys, sidx, scnt, pl.segcap, pl.nseg, 0};
+        const unsigned gc = (unsigned)std::min<u64>(2048, pl.nseg);
         if (radix) {
             for (int d = 0; d <= D0; ++d) {
-                if (!(d == 0 && fuse_hist)) HIPCHK(launch_hist(0, ckeys, sec, hist, bins, gc, st));
+                if (!(d == 0 && fuse_hist)) HIPCHK(launch_hist_seg(0, cseg, sec, hist, bins, gc, st));
                 HIPCHK(launch_select(0, hist, bins, sec, ctrl, st));
             }
         } else {
-            HIPCHK(launch_minmax(ckeys, sec, ctrl, gc, st));
+            HIPCHK(launch_minmax_seg(cseg, sec, gc, st));
             HIPCHK(launch_bucket_begin(sec, ctrl, 0, st));
             for (int l = 0; l < Lmax; ++l) {
-                HIPCHK(launch_hist(1, ckeys, sec, hist, bins, gc, st));
+                HIPCHK(launch_hist_seg(1, cseg, sec, hist, bins, gc, st));
                 HIPCHK(launch_select(1, hist, bins, sec, ctrl, st));
             }
-            HIPCHK(launch_find(ckeys, 0, sec, gc, st));
+            HIPCHK(launch_find_seg(cseg, sec, gc, st));
         }
         HIPCHK(mark(4));
         if (want_out) {
-            ExtractArgs ea{ckeys, cidx, 0, pl.f32, pl.cap, pl.k, sec, ctrl, d_topk_keys, d_topk_idx, st_c,
-                           &ctrl->tickets[2]};
+            ExtractArgs ea{cseg, 0, pl.f32, pl.k, sec, ctrl, d_topk_keys, d_topk_idx, ecnt, eoff};
             HIPCHK(launch_extract(ea, st));
         }
         HIPCHK(launch_finalize(sec, ctrl, pl.f32, d_kth, pl.k, st));

@@ -18,6 +18,11 @@ namespace drtopk {
 
 constexpr int WAVE = 64;       // CDNA wavefront width
 constexpr int WG = 256;        // default workgroup: 4 waves
+// Histograms of <= 256 bins are kept as HIST_REPS replicas in global memory: workgroup b
+// flushes into replica b % HIST_REPS and the select step sums them, so a skewed digit
+// does not serialise every flush on one L2 address.
+constexpr int HIST_REPS = 16;
+__host__ __device__ __forceinline__ int hist_reps(int bins) { return bins <= 256 ? HIST_REPS : 1; }
 
 // ---------------------------------------------------------------- key transforms
 struct KeyU32 {
@@ -69,12 +74,10 @@ struct Ctrl {
     u64 ties;             // keys == k-th largest in the top-k set
     u32 kth;              // ordered bits of the k-th largest
     u32 error;            // nonzero: a device-side check failed
-    u32 tickets[8];       // dynamic tile tickets of the look-back kernels
-    u32 pad[6];
+    u32 pad[14];
 };
 
 enum : u32 {
-    ERR_LOOKBACK_SPIN = 1u,   // a look-back wait ran out of its spin budget
     ERR_CONCAT_SHORT = 2u,    // concatenation smaller than k (cannot happen for valid input)
     ERR_CAPACITY = 4u         // a buffer bound was exceeded
 };
@@ -141,61 +144,26 @@ __device__ __forceinline__ u64 block_excl_scan_u64(u64 v, u64* scratch, u64* tot
     return off + inc - v;
 }
 
-// ---------------------------------------------------------------- decoupled look-back
-// Tiles are taken in ticket order (an atomic counter), so a tile only ever waits for
-// tiles that were taken before it, whose workgroups are already running: the wait
-// always ends.  As a second guard every wait has a spin budget; running out sets
-// Ctrl::error and continues, so no kernel can hang.
-// status word: bits 62..63 = 0 not ready, 1 aggregate only, 2 inclusive prefix;
-// bits 0..61 = payload (a count, or two 31-bit counts packed hi<<31|lo).
-constexpr u64 LB_AGG = 1ull << 62, LB_INC = 2ull << 62, LB_PAYLOAD = (1ull << 62) - 1;
-constexpr u32 LB_SPIN_LIMIT = 1u << 22;
-
-__device__ __forceinline__ u64 lb_load(const u64* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void lb_store(u64* p, u64 v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Called by ONE thread of the tile.  Returns the exclusive prefix of `aggregate`.
-__device__ __forceinline__ u64 lookback(u64* status, u32 tile, u64 aggregate, u32* err) {
-    if (tile == 0) {
-        lb_store(&status[0], LB_INC | aggregate);
-        return 0;
+// Whole-wave histogram update: lanes with `active` add bin `b`.  The most common bins
+// are peeled with one LDS atomic each (two rounds), the rest go one by one.
+__device__ __forceinline__ void wave_hist_add(u32* lhist, bool active, u32 b) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    bool rem = active;
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const u64 act = __ballot(rem);
+        if (!act) return;
+        const int src = __ffsll((unsigned long long)act) - 1;
+        const u32 pb = (u32)__shfl((int)b, src, WAVE);
+        const bool mine = rem && b == pb;
+        const u64 same = __ballot(mine);
+        if (lane == src) atomicAdd(&lhist[pb], (u32)__popcll(same));
+        if (mine) rem = false;
     }
-    lb_store(&status[tile], LB_AGG | aggregate);
-    u64 excl = 0;
-    long t = (long)tile - 1;
-    u32 spins = 0;
-    while (t >= 0) {
-        u64 s = lb_load(&status[t]);
-        while ((s >> 62) == 0) {
-            if (++spins > LB_SPIN_LIMIT) {
-                atomicOr(err, ERR_LOOKBACK_SPIN);
-                s = LB_INC;  // give up: treat as an inclusive zero
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            s = lb_load(&status[t]);
-        }
-        excl += s & LB_PAYLOAD;
-        if ((s >> 62) == 2) break;
-        --t;
-    }
-    lb_store(&status[tile], LB_INC | (excl + aggregate));
-    return excl;
+    if (rem) atomicAdd(&lhist[b], 1u);
 }
 
-// Ticket for the next tile (thread 0 draws, the block reads it from LDS).
-__device__ __forceinline__ u32 next_tile(u32* ticket, u32* lds_slot) {
-    if (threadIdx.x == 0) *lds_slot = atomicAdd(ticket, 1u);
-    __syncthreads();
-    u32 t = *lds_slot;
-    __syncthreads();
-    return t;
-}
-
+// ---------------------------------------------------------------- packed counts
 // Pack two 31-bit counts for the look-back payload.
 __device__ __forceinline__ u64 pack2(u64 hi, u64 lo) { return (hi << 31) | lo; }
 __device__ __forceinline__ u64 unpack_hi(u64 p) { return (p >> 31) & 0x7FFFFFFFull; }

@@ -1,0 +1,688 @@
+// compact.hip -- range selection, concatenation and top-k extraction of Dr.Top-k.
+//
+// Every ordered compaction here runs as
+//     count pass (workgroups independent)  ->  one-workgroup scan  ->  write pass
+// with no workgroup ever waiting on another.  (A single-pass decoupled look-back was
+// measured first: with ~1500 tiles in flight and ~2 us per cross-XCD status round trip,
+// each tile spent tens of microseconds walking back; the passes below do not wait.)
+//
+// The concatenation is kept as a segmented array: tile t of the range/gather pass writes
+// its survivors, in order, at [t*segcap, t*segcap + cnt[t]).  The second top-k reads the
+// segments directly, and the extraction's own count/scan/write orders them globally, so
+// the survivors are written once and never copied.
+//
+//  k_range_count / k_range_write  FindTopKRanges + FillWithKElementsID
+//        (baseline+filter+beta+shuffle/radixselect.cuh:413-427, :603-618): delegates whose
+//        prefix exceeds the first top-k flag, then the lowest-id delegates equal to it, up
+//        to k ids in ascending order (the reference's fill is a racy first-come append).
+//  k_gather  CreateConcatenateRange[_optimized] (radixselect.cuh:1546-1607): keys of the
+//        selected subranges (>= threshold with the filter) with their indices.
+//  k_beta_gather  FindTopKRanges_beta + CreateConcatenateRange_Beta_optimized
+//        (radixselect.cuh:472-600, :1610-1639) fused: a subrange whose beta delegates are
+//        all >= threshold contributes its keys, one whose maximum alone reaches it
+//        contributes its delegates ("lonely").
+//  k_extract_count / k_extract_write  findTopk + FillTopk (MultiGPU/radixselect.cuh:
+//        1355-1383): keys above the k-th largest plus the lowest-index keys equal to it.
+#include <algorithm>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace drtopk {
+
+constexpr int TE = 16;             // consecutive elements per thread
+constexpr u64 TT = WG * TE;        // 4096 elements per tile / chunk
+constexpr int SCAN_T = 1024;
+constexpr u64 CGRID = 8192;        // persistent grid bound of the count/write passes
+
+// ---------------------------------------------------------------- scan
+__global__ __launch_bounds__(SCAN_T) void k_scan(const u64* __restrict__ in, u64 nseg, u64* __restrict__ out,
+                                                 u64* total) {
+    __shared__ u64 scratch[SCAN_T / WAVE + 1];
+    const u64 per = (nseg + SCAN_T - 1) / SCAN_T;
+    const u64 b0 = std::min<u64>(nseg, threadIdx.x * per), b1 = std::min<u64>(nseg, b0 + per);
+    u64 sum = 0;
+    for (u64 i = b0; i < b1; ++i) sum += in[i];
+    u64 tot;
+    u64 run = block_excl_scan_u64<SCAN_T>(sum, scratch, &tot);
+    for (u64 i = b0; i < b1; ++i) {
+        const u64 v = in[i];
+        out[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+
+hipError_t launch_scan(const u64* in, u64 nseg, u64* out_excl, u64* total, hipStream_t st) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_T), 0, st, in, nseg, out_excl, total);
+    return hipGetLastError();
+}
+
+// Load the TE consecutive elements of this thread starting at p (16-byte aligned), zero past m.
+template <class K, bool RAW>
+__device__ __forceinline__ void load_te(const u32* p, u64 e0, u64 m, u32* v) {
+    if (e0 + TE <= m) {
+        const u32x4* q = (const u32x4*)p;
+#pragma unroll
+        for (int j = 0; j < TE / 4; ++j) {
+            const u32x4 r = q[j];
+            v[4 * j] = RAW ? K::to_ord(r.x) : r.x;
+            v[4 * j + 1] = RAW ? K::to_ord(r.y) : r.y;
+            v[4 * j + 2] = RAW ? K::to_ord(r.z) : r.z;
+            v[4 * j + 3] = RAW ? K::to_ord(r.w) : r.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < TE; ++j) v[j] = e0 + j < m ? (RAW ? K::to_ord(p[j]) : p[j]) : 0u;
+    }
+}
+
+// ---------------------------------------------------------------- LDS staging
+// Survivors of a 4096-element chunk go to LDS at their rank, then the workgroup copies the
+// chunk's run out with coalesced stores (a thread's own survivors sit at scattered global
+// positions, so storing them directly costs one cache line per lane per store).
+constexpr u32 STAGE = 4096;
+
+template <bool IDX, bool HIST2>
+__device__ __forceinline__ void flush_stage(const u32* sk, const u32* si, u32 cnt, u32* __restrict__ gk,
+                                            u32* __restrict__ gi, u64 gpos, u32* lh, int hshift2) {
+    for (u32 b = 0; b < cnt; b += WG) {  // uniform bound: every lane reaches wave_hist_add
+        const u32 i = b + threadIdx.x;
+        const bool act = i < cnt;
+        u32 v = 0;
+        if (act) {
+            v = sk[i];
+            gk[gpos + i] = v;
+            if (IDX) gi[gpos + i] = si[i];
+        }
+        if (HIST2) wave_hist_add(lh, act, v >> hshift2);
+    }
+}
+
+// ---------------------------------------------------------------- range selection (default path)
+struct RangeRule {
+    u32 thr, mask;
+    __device__ __forceinline__ RangeRule(const SelState* sel, int fullcmp) {
+        thr = sel->result;
+        const int shleft = fullcmp ? 0 : (sel->digit + 1) * (int)sel->nb;
+        mask = shleft >= 32 ? 0u : ~((1u << shleft) - 1u);
+    }
+};
+
+__global__ __launch_bounds__(WG) void k_range_count(const u32* __restrict__ dvals, u64 S, int fullcmp, SelState* sel,
+                                                    u64* __restrict__ cnt) {
+    __shared__ u64 scratch[WG / WAVE + 1];
+    const RangeRule rr(sel, fullcmp);
+    const u64 ntiles = range_tiles(S);
+    for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const u64 e0 = t * TT + (u64)threadIdx.x * TE;
+        u32 v[TE];
+        load_te<KeyU32, false>(dvals + e0, e0, S, v);
+        u32 gt = 0, eq = 0;
+#pragma unroll
+        for (int j = 0; j < TE; ++j) {
+            const bool in = e0 + j < S;
+            const u32 p = v[j] & rr.mask;
+            gt += in && p > rr.thr;
+            eq += in && p == rr.thr;
+        }
+        u64 tot;
+        block_excl_scan_u64<WG>(pack2(gt, eq), scratch, &tot);
+        if (threadIdx.x == 0) cnt[t] = tot;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_range_write(const u32* __restrict__ dvals, u64 S, u64 k, int fullcmp,
+                                                    SelState* sel, const u64* __restrict__ off, u32* __restrict__ ids,
+                                                    Ctrl* ctrl) {
+    __shared__ u64 scratch[WG / WAVE + 1];
+    __shared__ u32 sk[STAGE];
+    const RangeRule rr(sel, fullcmp);
+    const u64 total_gt = sel->m - sel->below - sel->kcount;
+    const u64 r1 = k - total_gt;  // delegates equal to the flag that are taken, lowest ids first
+    const u64 ntiles = range_tiles(S);
+    for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const u64 e0 = t * TT + (u64)threadIdx.x * TE;
+        u32 v[TE];
+        load_te<KeyU32, false>(dvals + e0, e0, S, v);
+        u32 gt = 0, eq = 0;
+#pragma unroll
+        for (int j = 0; j < TE; ++j) {
+            const bool in = e0 + j < S;
+            v[j] &= rr.mask;
+            gt += in && v[j] > rr.thr;
+            eq += in && v[j] == rr.thr;
+        }
+        u64 tot;
+        const u64 ex = block_excl_scan_u64<WG>(pack2(gt, eq), scratch, &tot);
+        const u64 base = off[t];
+        // the tile's ids occupy [P0, P1): every id before position P0 comes from earlier tiles
+        const u64 P0 = unpack_hi(base) + std::min<u64>(unpack_lo(base), r1);
+        const u64 P1 = unpack_hi(base) + unpack_hi(tot) + std::min<u64>(unpack_lo(base) + unpack_lo(tot), r1);
+        u64 g = unpack_hi(base) + unpack_hi(ex), q = unpack_lo(base) + unpack_lo(ex);
+#pragma unroll
+        for (int j = 0; j < TE; ++j) {
+            if (e0 + j >= S) break;
+            u64 pos = ~0ull;
+            if (v[j] > rr.thr) {
+                pos = g + (q < r1 ? q : r1);
+                ++g;
+            } else if (v[j] == rr.thr) {
+                if (q < r1) pos = g + q;
+                ++q;
+            }
+            if (pos != ~0ull) sk[pos - P0] = (u32)(e0 + j);
+        }
+        __syncthreads();
+        const u32 c = (u32)(P1 - P0);
+        if (P1 > k) {
+            if (threadIdx.x == 0) atomicOr(&ctrl->error, ERR_CAPACITY);
+        } else {
+            for (u32 i = threadIdx.x; i < c; i += WG) ids[P0 + i] = sk[i];
+        }
+        __syncthreads();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->selected = k;
+}
+
+hipError_t launch_range(const RangeArgs& a, hipStream_t st) {
+    const u64 nt = range_tiles(a.S);
+    const unsigned grid = (unsigned)std::min<u64>(nt, CGRID);
+    hipLaunchKernelGGL(k_range_count, dim3(grid), dim3(WG), 0, st, a.dvals, a.S, a.fullcmp, a.sel, a.cnt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = launch_scan(a.cnt, nt, a.off, nullptr, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_range_write, dim3(grid), dim3(WG), 0, st, a.dvals, a.S, a.k, a.fullcmp, a.sel, a.off,
+                       a.sel_ids, a.ctrl);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- histogram fusion helpers
+template <bool HIST2>
+__device__ __forceinline__ void h2_init(u32* lh, int bins2) {
+    if (HIST2)
+        for (int i = threadIdx.x; i < bins2; i += WG) lh[i] = 0;
+    __syncthreads();
+}
+template <bool HIST2>
+__device__ __forceinline__ void h2_flush(u32* lh, u32* ghist2, int bins2) {
+    if (HIST2) {
+        __syncthreads();
+        u32* g = ghist2 + (size_t)(blockIdx.x % hist_reps(bins2)) * bins2;
+        for (int i = threadIdx.x; i < bins2; i += WG) {
+            const u32 c = lh[i];
+            if (c) atomicAdd(&g[i], c);
+        }
+    }
+}
+
+// Chunks per segment: enough segments to fill the chip (~2048), at most 16 chunks each.
+u64 chunks_per_segment(u64 nchunks) { return std::max<u64>(1, std::min<u64>(16, nchunks / 2048)); }
+
+// ---------------------------------------------------------------- default-path concatenation
+template <class K, bool HIST2, bool IDX>
+__global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64 n, int alpha, int filter, int vec_ok,
+                                               const u32* __restrict__ sel_ids, u64 nsel, u64 cps, Ctrl* ctrl,
+                                               u32* __restrict__ skeys, u32* __restrict__ sidx, u32* __restrict__ cnt,
+                                               u32* ghist2, int hshift2, int bins2) {
+    __shared__ u64 scratch[WG / WAVE + 1];
+    __shared__ u32 lh[HIST2 ? 256 : 1];
+    __shared__ u32 sk[STAGE];
+    __shared__ u32 si[IDX ? STAGE : 1];
+    h2_init<HIST2>(lh, bins2);
+    const u32 thr = ctrl->first.result;
+    const u64 L = 1ull << alpha, Lm = L - 1;
+    const u64 V = nsel << alpha;
+    const u64 nchunks = (V + TT - 1) / TT;
+    const u64 nseg = (nchunks + cps - 1) / cps;
+    const u64 segcap = cps * TT;
+    for (u64 s = blockIdx.x; s < nseg; s += gridDim.x) {
+        u64 run = 0;
+        for (u64 ch = s * cps; ch < std::min<u64>((s + 1) * cps, nchunks); ++ch) {
+            const u64 v0 = ch * TT + (u64)threadIdx.x * TE;
+            u32 kv[TE];
+            u64 g0[TE / 4];
+            u32 take = 0;  // bit j: element j passes
+#pragma unroll
+            for (int q = 0; q < TE / 4; ++q) {
+                const u64 vp = v0 + 4 * q;
+                u32 r[4] = {0, 0, 0, 0};
+                u64 gq = 0;
+                if (vp < V && L >= 4) {
+                    gq = ((u64)sel_ids[vp >> alpha] << alpha) + (vp & Lm);
+                    if (vec_ok && gq + 3 < n) {
+                        const u32x4 x = *(const u32x4*)(keys + gq);
+                        r[0] = x.x; r[1] = x.y; r[2] = x.z; r[3] = x.w;
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            if (gq + c < n) r[c] = keys[gq + c];
+                    }
+                }
+                g0[q] = gq;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const u64 vpc = vp + c;
+                    u64 g = gq + c;
+                    bool ok = vpc < V;
+                    if (ok && L < 4) {
+                        g = ((u64)sel_ids[vpc >> alpha] << alpha) + (vpc & Lm);
+                        if (g < n) r[c] = keys[g];
+                    }
+                    ok = ok && g < n;
+                    const u32 o = K::to_ord(r[c]);
+                    kv[4 * q + c] = o;
+                    if (ok && (!filter || o >= thr)) take |= 1u << (4 * q + c);
+                }
+            }
+            u64 tot;
+            u32 slot = (u32)block_excl_scan_u64<WG>((u64)__popc(take), scratch, &tot);
+#pragma unroll
+            for (int j = 0; j < TE; ++j) {
+                if (take & (1u << j)) {
+                    sk[slot] = kv[j];
+                    if (IDX) {
+                        u64 g = g0[j >> 2] + (j & 3);
+                        if (L < 4) {
+                            const u64 vpc = v0 + j;
+                            g = ((u64)sel_ids[vpc >> alpha] << alpha) + (vpc & Lm);
+                        }
+                        si[slot] = (u32)g;
+                    }
+                    ++slot;
+                }
+            }
+            __syncthreads();
+            flush_stage<IDX, HIST2>(sk, si, (u32)tot, skeys, sidx, s * segcap + run, lh, hshift2);
+            __syncthreads();
+            run += tot;
+        }
+        if (threadIdx.x == 0) cnt[s] = (u32)run;
+    }
+    h2_flush<HIST2>(lh, ghist2, bins2);
+}
+
+hipError_t launch_gather(const GatherArgs& a, hipStream_t st) {
+    const u64 V = a.nsel << a.alpha;
+    const u64 nch = (V + TT - 1) / TT;
+    const u64 cps = chunks_per_segment(nch);
+    const u64 nseg = (nch + cps - 1) / cps;
+    const unsigned grid = (unsigned)std::min<u64>(nseg, CGRID);
+    const int vec_ok = ((uintptr_t)a.keys & 15) == 0;
+#define G_ARGS a.keys, a.n, a.alpha, a.filter, vec_ok, a.sel_ids, a.nsel, cps, a.ctrl, a.skeys, a.sidx, a.cnt, \
+               a.ghist2, a.hshift2, a.bins2
+#define G_LAUNCH(K, H, I) hipLaunchKernelGGL((k_gather<K, H, I>), dim3(grid), dim3(WG), 0, st, G_ARGS)
+    const bool h = a.ghist2 != nullptr, i = a.sidx != nullptr;
+    if (a.f32) {
+        if (h && i) G_LAUNCH(KeyF32, true, true); else if (h) G_LAUNCH(KeyF32, true, false);
+        else if (i) G_LAUNCH(KeyF32, false, true); else G_LAUNCH(KeyF32, false, false);
+    } else {
+        if (h && i) G_LAUNCH(KeyU32, true, true); else if (h) G_LAUNCH(KeyU32, true, false);
+        else if (i) G_LAUNCH(KeyU32, false, true); else G_LAUNCH(KeyU32, false, false);
+    }
+#undef G_LAUNCH
+#undef G_ARGS
+    return hipGetLastError();
+}
+u64 gather_segcap(u64 V) { return chunks_per_segment((V + TT - 1) / TT) * TT; }
+u64 gather_nseg(u64 V) {
+    const u64 nch = (V + TT - 1) / TT;
+    const u64 cps = chunks_per_segment(nch);
+    return (nch + cps - 1) / cps;
+}
+
+// ---------------------------------------------------------------- beta-path range selection + concatenation
+constexpr int BSPT = 4;  // subranges per thread (1024 per chunk)
+
+template <class K, int BETA, bool POS, bool HIST2>
+__global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys, u64 n, int alpha, int filter,
+                                                    const u32* __restrict__ dvals, const void* __restrict__ doff,
+                                                    u64 cps, Ctrl* ctrl, u32* __restrict__ skeys,
+                                                    u32* __restrict__ sidx, u32* __restrict__ cnt, u32* ghist2,
+                                                    int hshift2, int bins2) {
+    __shared__ u64 scratch[WG / WAVE + 1];
+    __shared__ u32 lh[HIST2 ? 256 : 1];
+    __shared__ u64 red[3][WG / WAVE];
+    __shared__ u32 sk[STAGE];
+    __shared__ u32 si[POS ? STAGE : 1];
+    h2_init<HIST2>(lh, bins2);
+    const u32 thr = ctrl->first.result;
+    const u64 L = 1ull << alpha;
+    const u64 S = (n + L - 1) >> alpha;
+    const u64 nchunks = (S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS;
+    const u64 nseg = (nchunks + cps - 1) / cps;
+    const u64 segcap = cps * (BETA_TILE_SUBS << alpha);
+    const bool vec_keys = alpha >= 2;
+    u64 nsel = 0, nlon = 0, nlel = 0;
+    auto count_sel = [&](u64 base, u64 len) -> u32 {
+        if (!filter) return (u32)len;
+        u32 c = 0;
+        if (vec_keys && len == L) {
+            const u32x4* p = (const u32x4*)(keys + base);
+            for (u64 q = 0; q < (L >> 2); ++q) {
+                const u32x4 r = p[q];
+                c += (K::to_ord(r.x) >= thr) + (K::to_ord(r.y) >= thr) + (K::to_ord(r.z) >= thr) +
+                     (K::to_ord(r.w) >= thr);
+            }
+        } else {
+            for (u64 i = base; i < base + len; ++i) c += K::to_ord(keys[i]) >= thr;
+        }
+        return c;
+    };
+    for (u64 sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+        u64 run = 0;
+        for (u64 ch = sg * cps; ch < std::min<u64>((sg + 1) * cps, nchunks); ++ch) {
+            const u64 s0 = ch * BETA_TILE_SUBS + (u64)threadIdx.x * BSPT;
+            u32 d[BSPT][BETA];
+            if (s0 + BSPT <= S) {
+                const u32x4* p = (const u32x4*)(dvals + s0 * BETA);  // 4*BETA words, 16-byte aligned
+                u32 w[BSPT * BETA];
+#pragma unroll
+                for (int q = 0; q < BETA; ++q) {
+                    const u32x4 r = p[q];
+                    w[4 * q] = r.x; w[4 * q + 1] = r.y; w[4 * q + 2] = r.z; w[4 * q + 3] = r.w;
+                }
+#pragma unroll
+                for (int j = 0; j < BSPT; ++j)
+#pragma unroll
+                    for (int b = 0; b < BETA; ++b) d[j][b] = w[j * BETA + b];
+            } else {
+#pragma unroll
+                for (int j = 0; j < BSPT; ++j)
+#pragma unroll
+                    for (int b = 0; b < BETA; ++b) d[j][b] = s0 + j < S ? dvals[(s0 + j) * BETA + b] : 0u;
+            }
+            int kind[BSPT];  // 0 none, 1 selected (its keys), 2 lonely (its delegates)
+            u32 c = 0;
+#pragma unroll
+            for (int j = 0; j < BSPT; ++j) {
+                const u64 s = s0 + j;
+                kind[j] = 0;
+                if (s >= S || d[j][0] < thr) continue;
+                const u64 base = s << alpha;
+                const u64 len = std::min<u64>(L, n - base);
+                const int m = (int)std::min<u64>(BETA, len);
+                bool all = true;
+#pragma unroll
+                for (int b = 0; b < BETA; ++b) all = all && (b >= m || d[j][b] >= thr);
+                if (all) {
+                    kind[j] = 1;
+                    ++nsel;
+                    c += count_sel(base, len);
+                } else {
+                    kind[j] = 2;
+                    ++nlon;
+                    nlel += (u64)m;
+                    c += (u32)m;
+                }
+            }
+            u64 tot;
+            const u64 ex = block_excl_scan_u64<WG>((u64)c, scratch, &tot);
+            const bool staged = tot <= STAGE;  // uniform
+            const u64 gbase = sg * segcap + run;
+            u64 slot = ex;
+            auto put = [&](u32 o, u32 gidx) {
+                if (staged) {
+                    sk[slot] = o;
+                    if (POS) si[slot] = gidx;
+                } else {
+                    skeys[gbase + slot] = o;
+                    if (POS) sidx[gbase + slot] = gidx;
+                    if (HIST2) atomicAdd(&lh[o >> hshift2], 1u);
+                }
+                ++slot;
+            };
+#pragma unroll
+            for (int j = 0; j < BSPT; ++j) {
+                if (kind[j] == 0) continue;
+                const u64 s = s0 + j;
+                const u64 base = s << alpha;
+                const u64 len = std::min<u64>(L, n - base);
+                if (kind[j] == 1) {
+                    for (u64 i = base; i < base + len; ++i) {
+                        const u32 o = K::to_ord(keys[i]);
+                        if (!filter || o >= thr) put(o, (u32)i);
+                    }
+                } else {
+                    const int m = (int)std::min<u64>(BETA, len);
+                    u32 dv[BETA], p[BETA];
+                    u32 packed = 0;
+                    if (POS) {
+                        if (BETA == 2) packed = ((const uint8_t*)doff)[s];
+                        else if (BETA == 3) packed = ((const uint16_t*)doff)[s];
+                        else packed = ((const u32*)doff)[s];
+                    }
+#pragma unroll
+                    for (int b = 0; b < BETA; ++b) {
+                        dv[b] = d[j][b];
+                        const bool ge = b < m && dv[b] >= thr;  // only these can reach the output
+                        p[b] = (POS && ge && b < BETA - 1) ? (u32)(base + ((packed >> (8 * b)) & 0xFFu)) : 0xFFFFFFFFu;
+                    }
+                    // delegates >= threshold in index order, then the rest (never selected later)
+#pragma unroll
+                    for (int x = 0; x < BETA; ++x)
+#pragma unroll
+                        for (int y = 0; y + 1 < BETA - x; ++y)
+                            if (p[y + 1] < p[y]) {
+                                const u32 tp = p[y]; p[y] = p[y + 1]; p[y + 1] = tp;
+                                const u32 td = dv[y]; dv[y] = dv[y + 1]; dv[y + 1] = td;
+                            }
+#pragma unroll
+                    for (int b = 0; b < BETA; ++b)
+                        if (b < m) put(dv[b], p[b]);
+                }
+            }
+            __syncthreads();
+            if (staged) flush_stage<POS, HIST2>(sk, si, (u32)tot, skeys, sidx, gbase, lh, hshift2);
+            __syncthreads();
+            run += tot;
+        }
+        if (threadIdx.x == 0) cnt[sg] = (u32)run;
+    }
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            nsel += shfl_xor_u64(nsel, o);
+            nlon += shfl_xor_u64(nlon, o);
+            nlel += shfl_xor_u64(nlel, o);
+        }
+        if (lane == 0) { red[0][w] = nsel; red[1][w] = nlon; red[2][w] = nlel; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            u64 a = 0, b = 0, cc = 0;
+            for (int i = 0; i < WG / WAVE; ++i) { a += red[0][i]; b += red[1][i]; cc += red[2][i]; }
+            if (a) atomicAdd((unsigned long long*)&ctrl->selected, (unsigned long long)a);
+            if (b) atomicAdd((unsigned long long*)&ctrl->lonely_sub, (unsigned long long)b);
+            if (cc) atomicAdd((unsigned long long*)&ctrl->lonely_el, (unsigned long long)cc);
+        }
+    }
+    h2_flush<HIST2>(lh, ghist2, bins2);
+}
+
+u64 beta_nseg(u64 S) {
+    const u64 nch = (S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS;
+    const u64 cps = chunks_per_segment(nch);
+    return (nch + cps - 1) / cps;
+}
+u64 beta_segcap(u64 S, int alpha) {
+    return chunks_per_segment((S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS) * (BETA_TILE_SUBS << alpha);
+}
+
+template <class K, int BETA>
+static hipError_t launch_beta_t(const BetaGatherArgs& a, hipStream_t st) {
+    const u64 L = 1ull << a.alpha;
+    const u64 S = (a.n + L - 1) >> a.alpha;
+    const u64 nch = (S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS;
+    const u64 cps = chunks_per_segment(nch);
+    const u64 nseg = (nch + cps - 1) / cps;
+    const unsigned grid = (unsigned)std::min<u64>(nseg, CGRID);
+#define B_ARGS a.keys, a.n, a.alpha, a.filter, a.dvals, a.doff, cps, a.ctrl, a.skeys, a.sidx, a.cnt, a.ghist2, \
+               a.hshift2, a.bins2
+    const bool pos = a.sidx != nullptr, h = a.ghist2 != nullptr;
+    if (pos && h) hipLaunchKernelGGL((k_beta_gather<K, BETA, true, true>), dim3(grid), dim3(WG), 0, st, B_ARGS);
+    else if (pos) hipLaunchKernelGGL((k_beta_gather<K, BETA, true, false>), dim3(grid), dim3(WG), 0, st, B_ARGS);
+    else if (h) hipLaunchKernelGGL((k_beta_gather<K, BETA, false, true>), dim3(grid), dim3(WG), 0, st, B_ARGS);
+    else hipLaunchKernelGGL((k_beta_gather<K, BETA, false, false>), dim3(grid), dim3(WG), 0, st, B_ARGS);
+#undef B_ARGS
+    return hipGetLastError();
+}
+template <class K>
+static hipError_t launch_beta_k(const BetaGatherArgs& a, hipStream_t st) {
+    switch (a.beta) {
+        case 2: return launch_beta_t<K, 2>(a, st);
+        case 3: return launch_beta_t<K, 3>(a, st);
+        case 4: return launch_beta_t<K, 4>(a, st);
+    }
+    return hipErrorInvalidValue;
+}
+hipError_t launch_beta_gather(const BetaGatherArgs& a, hipStream_t st) {
+    return a.f32 ? launch_beta_k<KeyF32>(a, st) : launch_beta_k<KeyU32>(a, st);
+}
+
+// ---------------------------------------------------------------- concatenation size
+__global__ __launch_bounds__(SCAN_T) void k_seg_total(const u32* __restrict__ cnt, u64 nseg, Ctrl* ctrl) {
+    __shared__ u64 scratch[SCAN_T / WAVE + 1];
+    u64 s = 0;
+    for (u64 i = threadIdx.x; i < nseg; i += SCAN_T) s += cnt[i];
+    u64 tot;
+    block_excl_scan_u64<SCAN_T>(s, scratch, &tot);
+    if (threadIdx.x == 0) {
+        ctrl->concat_size = tot;
+        ctrl->second.m = tot;
+        if (tot < ctrl->second.k) {
+            atomicOr(&ctrl->error, ERR_CONCAT_SHORT);
+            ctrl->second.done = 1;  // nothing sensible to select; the call reports the error
+        }
+    }
+}
+hipError_t launch_seg_total(const u32* cnt, u64 nseg, Ctrl* ctrl, hipStream_t st) {
+    hipLaunchKernelGGL(k_seg_total, dim3(1), dim3(SCAN_T), 0, st, cnt, nseg, ctrl);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- extraction
+template <bool DIRECT>
+__device__ __forceinline__ u64 seg_len(const Seg& g, u64 s) {
+    return DIRECT ? std::min<u64>(g.segcap, g.n - s * g.segcap) : (u64)g.cnt[s];
+}
+
+template <class K, bool DIRECT>
+__global__ __launch_bounds__(WG) void k_extract_count(Seg g, SelState* sel, u64* __restrict__ cnt) {
+    __shared__ u64 scratch[WG / WAVE + 1];
+    const u32 K0 = sel->result;
+    for (u64 s = blockIdx.x; s < g.nseg; s += gridDim.x) {
+        const u64 len = seg_len<DIRECT>(g, s);
+        const u32* base = g.keys + s * g.segcap;
+        u32 gt = 0, eq = 0;
+        for (u64 c0 = 0; c0 < len; c0 += TT) {
+            const u64 e0 = c0 + (u64)threadIdx.x * TE;
+            u32 v[TE];
+            load_te<K, DIRECT>(base + e0, e0, len, v);
+#pragma unroll
+            for (int j = 0; j < TE; ++j) {
+                const bool in = e0 + j < len;
+                gt += in && v[j] > K0;
+                eq += in && v[j] == K0;
+            }
+        }
+        u64 tot;
+        block_excl_scan_u64<WG>(pack2(gt, eq), scratch, &tot);
+        if (threadIdx.x == 0) cnt[s] = tot;
+    }
+}
+
+template <class K, bool DIRECT, bool KEYS, bool IDX>
+__global__ __launch_bounds__(WG) void k_extract_write(Seg g, SelState* sel, u64 k, const u64* __restrict__ off,
+                                                      u32* __restrict__ out_keys, i64* __restrict__ out_idx,
+                                                      Ctrl* ctrl) {
+    __shared__ u64 scratch[WG / WAVE + 1];
+    __shared__ u32 sk[KEYS ? STAGE : 1];
+    __shared__ u32 si[IDX ? STAGE : 1];
+    const u32 K0 = sel->result;
+    const u64 total_gt = sel->m - sel->below - sel->kcount;
+    const u64 r = k - total_gt;  // ties taken, lowest index first
+    for (u64 s = blockIdx.x; s < g.nseg; s += gridDim.x) {
+        const u64 len = seg_len<DIRECT>(g, s);
+        const u32* base = g.keys + s * g.segcap;
+        const u64 o = off[s];
+        u64 run_g = unpack_hi(o), run_q = unpack_lo(o);
+        for (u64 c0 = 0; c0 < len; c0 += TT) {
+            const u64 e0 = c0 + (u64)threadIdx.x * TE;
+            u32 v[TE];
+            load_te<K, DIRECT>(base + e0, e0, len, v);
+            u32 gt = 0, eq = 0;
+#pragma unroll
+            for (int j = 0; j < TE; ++j) {
+                const bool in = e0 + j < len;
+                gt += in && v[j] > K0;
+                eq += in && v[j] == K0;
+            }
+            u64 tot;
+            const u64 ex = block_excl_scan_u64<WG>(pack2(gt, eq), scratch, &tot);
+            // the chunk's outputs occupy [P0, P1) of the top-k arrays
+            const u64 P0 = run_g + std::min<u64>(run_q, r);
+            const u64 P1 = run_g + unpack_hi(tot) + std::min<u64>(run_q + unpack_lo(tot), r);
+            u64 gg = run_g + unpack_hi(ex), q = run_q + unpack_lo(ex);
+#pragma unroll
+            for (int j = 0; j < TE; ++j) {
+                if (e0 + j >= len) break;
+                u64 pos = ~0ull;
+                if (v[j] > K0) {
+                    pos = gg + (q < r ? q : r);
+                    ++gg;
+                } else if (v[j] == K0) {
+                    if (q < r) pos = gg + q;
+                    ++q;
+                }
+                if (pos != ~0ull) {
+                    const u32 sl = (u32)(pos - P0);
+                    if (KEYS) sk[sl] = K::from_ord(v[j]);
+                    if (IDX) si[sl] = DIRECT ? (u32)(s * g.segcap + e0 + j) : g.idx[s * g.segcap + e0 + j];
+                }
+            }
+            __syncthreads();
+            if (P1 > k) {
+                if (threadIdx.x == 0) atomicOr(&ctrl->error, ERR_CAPACITY);
+            } else {
+                const u32 c = (u32)(P1 - P0);
+                for (u32 i = threadIdx.x; i < c; i += WG) {
+                    if (KEYS) out_keys[P0 + i] = sk[i];
+                    if (IDX) out_idx[P0 + i] = (i64)si[i];
+                }
+            }
+            __syncthreads();
+            run_g += unpack_hi(tot);
+            run_q += unpack_lo(tot);
+        }
+    }
+}
+
+hipError_t launch_extract(const ExtractArgs& a, hipStream_t st) {
+    const unsigned grid = (unsigned)std::max<u64>(1, std::min<u64>(a.seg.nseg, CGRID));
+#define EXC(K, D) hipLaunchKernelGGL((k_extract_count<K, D>), dim3(grid), dim3(WG), 0, st, a.seg, a.sel, a.cnt)
+    if (a.f32) { if (a.direct) EXC(KeyF32, true); else EXC(KeyF32, false); }
+    else { if (a.direct) EXC(KeyU32, true); else EXC(KeyU32, false); }
+#undef EXC
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = launch_scan(a.cnt, a.seg.nseg, a.off, nullptr, st);
+    if (e != hipSuccess) return e;
+#define EXW(K, D, KK, II)                                                                                        \
+    hipLaunchKernelGGL((k_extract_write<K, D, KK, II>), dim3(grid), dim3(WG), 0, st, a.seg, a.sel, a.k, a.off,   \
+                       (u32*)a.out_keys, a.out_idx, a.ctrl)
+#define EXW2(K, D)                                                                                                \
+    do {                                                                                                          \
+        if (a.out_keys && a.out_idx) EXW(K, D, true, true);                                                       \
+        else if (a.out_keys) EXW(K, D, true, false);                                                              \
+        else EXW(K, D, false, true);                                                                              \
+    } while (0)
+    if (a.f32) { if (a.direct) EXW2(KeyF32, true); else EXW2(KeyF32, false); }
+    else { if (a.direct) EXW2(KeyU32, true); else EXW2(KeyU32, false); }
+#undef EXW2
+#undef EXW
+    return hipGetLastError();
+}
+
+}  // namespace drtopk

@@ -5,6 +5,17 @@
 
 namespace drtopk {
 
+// A segmented array: segment s holds len(s) elements at [s*segcap, s*segcap + len(s)).
+// len(s) = cnt[s], or min(segcap, n - s*segcap) when cnt is null (a flat array in tiles).
+struct Seg {
+    const u32* keys;
+    const u32* idx;
+    const u32* cnt;
+    u64 segcap;
+    u64 nseg;
+    u64 n;
+};
+
 struct SampleArgs {
     const u32* keys;   // raw keys (uint32 or float32 bits)
     u64 n;
@@ -28,76 +39,91 @@ hipError_t launch_hist(int mode, const u32* arr, SelState* sel, u32* ghist, int 
                        hipStream_t st);
 hipError_t launch_hist_raw(int mode, int f32, const u32* keys, SelState* sel, u32* ghist, int bins,
                            unsigned grid, hipStream_t st);
+hipError_t launch_hist_seg(int mode, const Seg& seg, SelState* sel, u32* ghist, int bins, unsigned grid,
+                           hipStream_t st);
 hipError_t launch_select(int mode, u32* ghist, int bins, SelState* sel, Ctrl* ctrl, hipStream_t st);
 hipError_t launch_bucket_begin(SelState* sel, Ctrl* ctrl, int from_ctrl_minmax, hipStream_t st);
 hipError_t launch_minmax(const u32* arr, SelState* sel, Ctrl* ctrl, unsigned grid, hipStream_t st);
 hipError_t launch_minmax_raw(int f32, const u32* keys, SelState* sel, Ctrl* ctrl, unsigned grid,
                              hipStream_t st);
+hipError_t launch_minmax_seg(const Seg& seg, SelState* sel, unsigned grid, hipStream_t st);
 hipError_t launch_find(const u32* arr, int raw_f32, SelState* sel, unsigned grid, hipStream_t st);
+hipError_t launch_find_seg(const Seg& seg, SelState* sel, unsigned grid, hipStream_t st);
 
+// compact.hip -------------------------------------------------------------------------
+// Ordered compactions run as count pass -> single-workgroup scan -> write pass, with no
+// waiting between workgroups.
+hipError_t launch_scan(const u64* in, u64 nseg, u64* out_excl, u64* total, hipStream_t st);
+
+// FindTopKRanges + FillWithKElementsID: exactly k subrange ids, ascending.
 struct RangeArgs {
     const u32* dvals;
-    u64 S;             // subranges
+    u64 S;
     u64 k;
-    int mode;          // first-top-k mode (0 radix prefix compare, 1 full compare)
+    int fullcmp;       // 1: compare full keys (bucket), 0: radix prefix of the first top-k
     SelState* sel;     // first top-k state
     Ctrl* ctrl;
-    u32* sel_ids;      // [k] selected subrange ids, ascending
-    u64* status;       // look-back status, one per tile
-    u32* ticket;
+    u32* sel_ids;      // [k]
+    u64* cnt;          // per-tile packed (greater, equal) counts
+    u64* off;          // their exclusive prefix
 };
-hipError_t launch_range_default(const RangeArgs& a, hipStream_t st);
+hipError_t launch_range(const RangeArgs& a, hipStream_t st);
+__host__ __device__ inline u64 range_tiles(u64 S) { return (S + 4095) / 4096; }
 
+// Default-path concatenation, staged per 4096-key tile of the selected subranges.
 struct GatherArgs {
-    const u32* keys;   // raw keys
+    const u32* keys;
     u64 n;
-    int alpha;
-    int f32;
-    int filter;
+    int alpha, f32, filter;
     const u32* sel_ids;
-    u64 nsel;          // == k on the default path
+    u64 nsel;
     Ctrl* ctrl;
-    u32* ckeys;        // concatenation (ordered keys)
-    u32* cidx;         // concatenation indices or nullptr
-    u64 cap;           // capacity of ckeys/cidx
-    u64* status;
-    u32* ticket;
-    u32* ghist2;       // optional fused top-digit histogram of the concatenation (second top-k)
+    u32* skeys;        // segment t at [t*GATHER_TILE, ...)
+    u32* sidx;         // or nullptr
+    u32* cnt;          // [nseg]
+    u32* ghist2;       // fused top-digit histogram of the concatenation, or nullptr
     int hshift2, bins2;
 };
-hipError_t launch_gather_default(const GatherArgs& a, hipStream_t st);
+hipError_t launch_gather(const GatherArgs& a, hipStream_t st);
+u64 gather_segcap(u64 V);   // segment capacity / count for V = nsel << alpha virtual keys
+u64 gather_nseg(u64 V);
 
+// Beta-path range selection + concatenation, staged per 1024-subrange tile.
 struct BetaGatherArgs {
     const u32* keys;
     u64 n;
     int alpha, beta, f32, filter;
     const u32* dvals;
-    const void* doff;  // packed offsets or nullptr
+    const void* doff;
     Ctrl* ctrl;
-    u32* ckeys;
-    u32* cidx;
-    u64 cap;
-    u64* status;
-    u32* ticket;
+    u32* skeys;        // segment t at [t*segcap, ...)
+    u32* sidx;
+    u32* cnt;
     u32* ghist2;
     int hshift2, bins2;
 };
-hipError_t launch_range_gather_beta(const BetaGatherArgs& a, hipStream_t st);
+constexpr u64 BETA_TILE_SUBS = 1024;
+hipError_t launch_beta_gather(const BetaGatherArgs& a, hipStream_t st);
+u64 beta_segcap(u64 S, int alpha);
+u64 beta_nseg(u64 S);
 
+// concatenation size = sum of segment counts -> ctrl->concat_size, second.m
+hipError_t launch_seg_total(const u32* cnt, u64 nseg, Ctrl* ctrl, hipStream_t st);
+
+// findTopk + FillTopk: keys above the k-th largest, then the lowest-index ties, index order.
 struct ExtractArgs {
-    const u32* ckeys;  // ordered keys (or raw keys when direct)
-    const u32* cidx;   // indices or nullptr (direct: position)
-    int direct;        // 1: extract straight from V (plain path), keys are raw
+    Seg seg;           // concatenation, or V itself (cnt == nullptr, raw keys)
+    int direct;        // 1: seg is V (raw keys, index = position)
     int f32;
-    u64 cap;           // bound on the array length (for the grid)
     u64 k;
-    SelState* sel;     // second top-k state (k-th largest)
+    SelState* sel;     // second top-k state
     Ctrl* ctrl;
     void* out_keys;
     i64* out_idx;
-    u64* status;
-    u32* ticket;
+    u64* cnt;          // [nseg] packed (greater, equal)
+    u64* off;
 };
+constexpr u64 DIRECT_TILE = 16384;
 hipError_t launch_extract(const ExtractArgs& a, hipStream_t st);
 hipError_t launch_finalize(SelState* sel, Ctrl* ctrl, int f32, void* d_kth, u64 k, hipStream_t st);
 hipError_t launch_to_user(const u32* ord, u64 m, int f32, void* out, hipStream_t st);

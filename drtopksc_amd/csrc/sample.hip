@@ -5,19 +5,23 @@
 // 2^alpha keys keep its maximum (beta == 1) or its top-beta multiset (beta 2..4),
 // largest first, lowest index first among equal keys.
 //
-// Three variants, identical output:
-//  - lane groups (alpha <= 8): a workgroup streams 8192 keys with 16-byte loads, lane l
-//    of a wave holding keys 4l..4l+3 of a 256-key wave tile; a subrange is a group of
-//    2^alpha/4 lanes reduced with xor shuffles (2 subranges per lane at alpha == 1).
-//  - wave per subrange (alpha >= 9): a wave strides one subrange with 16-byte loads and
-//    reduces over its 64 lanes.
+// Three variants, identical output, all persistent (grid-stride over work units):
+//  - lane groups (alpha <= 8): a workgroup streams 8192-key chunks with 16-byte loads,
+//    lane l of a wave holding keys 4l..4l+3 of a 256-key wave tile; a subrange is a group
+//    of 2^alpha/4 lanes reduced with xor shuffles (2 subranges per lane at alpha == 1).
+//  - wave units (alpha >= 9): a wave streams max(2^alpha, 2048) keys with 16-byte loads,
+//    eight in flight per lane, and reduces each subrange over its 64 lanes.
 //  - LDS staged, thread per subrange (alpha <= 5, DRTOPK_FLAG_SHUFFLE): the workgroup
 //    stages 256 subranges in LDS with coalesced loads, then each thread scans its own
-//    subrange (the reference's "shuffle" option).
+//    subrange with 16-byte LDS reads (the reference's "shuffle" option).
 //
 // Fused into the same pass: the histogram of the delegates' top digit (radix family,
-// the first digit of the first top-k) or the delegates' min/max (bucket family).
+// the first digit of the first top-k) or the delegates' min/max (bucket family).  The
+// histogram is aggregated per wave (the most common bin of the wave is counted with one
+// LDS atomic) and flushed once per workgroup into one of HIST_REPS global replicas, so
+// skewed digits do not serialise on a single L2 address.
 #include <algorithm>
+#include <type_traits>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -25,36 +29,41 @@
 namespace drtopk {
 
 // ---------------------------------------------------------------- top-beta lists
-// Element: u64 composite (ordered key << 32 | ~index) when positions are tracked, else the
-// ordered key (u32 widened).  Larger element = larger key, then lower index.  Absent = 0.
-template <int BETA>
+// E = u32: plain ordered keys (multiset of values, enough when positions are not needed).
+// E = u64: composite (ordered key << 32 | ~index): larger key wins, then lower index.
+// Absent = 0 (smaller than any composite of a real key, and equal to the pad value 0).
+template <int BETA, class E>
 struct TopList {
-    u64 v[BETA];
+    E v[BETA];
     __device__ __forceinline__ void clear() {
 #pragma unroll
         for (int j = 0; j < BETA; ++j) v[j] = 0;
     }
-    __device__ __forceinline__ void insert(u64 x) {
-        // keep v sorted descending; drop the smallest
+    __device__ __forceinline__ void insert(u32 ord, u64 position) { insert_e(elem(ord, position)); }
+    static __device__ __forceinline__ E elem(u32 ord, u64 position) {
+        if (sizeof(E) == 8) return (E)(((u64)ord << 32) | (u64)(0xFFFFFFFFu - (u32)position));
+        return (E)ord;
+    }
+    __device__ __forceinline__ void insert_e(E x) {
 #pragma unroll
         for (int j = 0; j < BETA; ++j) {
-            u64 hi = v[j] > x ? v[j] : x;
-            u64 lo = v[j] > x ? x : v[j];
+            E hi = v[j] > x ? v[j] : x;
+            E lo = v[j] > x ? x : v[j];
             v[j] = hi;
             x = lo;
         }
     }
-    // top-BETA of the union of two descending lists: max(a[i], b[BETA-1-i]) is a bitonic
-    // sequence holding the top BETA; a small sorting network orders it.
-    __device__ __forceinline__ void merge(const u64* b) {
-        u64 c[BETA];
+    // top-BETA of two descending lists: max(a[i], b[BETA-1-i]) is bitonic and holds the
+    // top BETA; a small sorting network orders it.
+    __device__ __forceinline__ void merge(const E* b) {
+        E c[BETA];
 #pragma unroll
         for (int j = 0; j < BETA; ++j) c[j] = v[j] > b[BETA - 1 - j] ? v[j] : b[BETA - 1 - j];
 #pragma unroll
         for (int p = 0; p < BETA; ++p)
 #pragma unroll
             for (int q = 0; q + 1 < BETA - p; ++q) {
-                u64 x = c[q], y = c[q + 1];
+                E x = c[q], y = c[q + 1];
                 c[q] = x > y ? x : y;
                 c[q + 1] = x > y ? y : x;
             }
@@ -62,41 +71,78 @@ struct TopList {
         for (int j = 0; j < BETA; ++j) v[j] = c[j];
     }
     __device__ __forceinline__ void shfl_merge(int off) {
-        u64 b[BETA];
+        E b[BETA];
 #pragma unroll
-        for (int j = 0; j < BETA; ++j) b[j] = shfl_xor_u64(v[j], off);
+        for (int j = 0; j < BETA; ++j) {
+            if (sizeof(E) == 8) b[j] = (E)shfl_xor_u64((u64)v[j], off);
+            else b[j] = (E)(u32)__shfl_xor((int)(u32)v[j], off, WAVE);
+        }
         merge(b);
+    }
+    __device__ __forceinline__ u32 val(int j) const { return sizeof(E) == 8 ? (u32)((u64)v[j] >> 32) : (u32)v[j]; }
+    __device__ __forceinline__ u32 pos(int j) const {
+        return (sizeof(E) == 8 && v[j] != 0) ? 0xFFFFFFFFu - (u32)(u64)v[j] : 0xFFFFFFFFu;
     }
 };
 
-template <bool COMPOSITE>
-__device__ __forceinline__ u64 elem(u32 ord, u64 pos) {
-    if (COMPOSITE) return ((u64)ord << 32) | (u64)(0xFFFFFFFFu - (u32)pos);
-    return (u64)ord;
-}
+// Top-2 by value with the index of the first maximum (the beta == 2 path with offsets):
+// m1 = max, m2 = second largest (multiset), p1 = lowest index holding m1.  Two u32 min/max
+// per key instead of the u64 composite network.
+struct Top2P {
+    u32 m1, m2, p1;
+    __device__ __forceinline__ void clear() { m1 = 0; m2 = 0; p1 = 0xFFFFFFFFu; }
+    // keys arrive in increasing index order within a lane
+    __device__ __forceinline__ void insert(u32 x, u64 position) {
+        const bool up = x > m1 || p1 == 0xFFFFFFFFu;
+        m2 = max(min(m1, x), m2);
+        if (p1 == 0xFFFFFFFFu) m2 = 0;  // first key: no second yet
+        p1 = up ? (u32)position : p1;
+        m1 = max(m1, x);
+    }
+    __device__ __forceinline__ void shfl_merge(int off) {
+        const u32 bm1 = (u32)__shfl_xor((int)m1, off, WAVE);
+        const u32 bm2 = (u32)__shfl_xor((int)m2, off, WAVE);
+        const u32 bp1 = (u32)__shfl_xor((int)p1, off, WAVE);
+        const u32 n2 = max(min(m1, bm1), max(m2, bm2));
+        const bool empty_a = p1 == 0xFFFFFFFFu, empty_b = bp1 == 0xFFFFFFFFu;
+        u32 np;
+        if (empty_a) np = bp1;
+        else if (empty_b) np = p1;
+        else np = m1 > bm1 ? p1 : (bm1 > m1 ? bp1 : min(p1, bp1));
+        // an empty side contributes nothing: the other side's pair stands
+        const u32 nm2 = empty_a ? bm2 : (empty_b ? m2 : n2);
+        m1 = max(m1, bm1);
+        m2 = nm2;
+        p1 = np;
+    }
+    __device__ __forceinline__ u32 val(int j) const { return j == 0 ? m1 : m2; }
+    __device__ __forceinline__ u32 pos(int j) const { return j == 0 ? p1 : 0xFFFFFFFFu; }
+};
 
-// Write subrange s's delegates (+ positions / packed offsets) and feed the fused reduction.
+template <int BETA, int POSM>
+struct ListOf {
+    using type = TopList<BETA, typename std::conditional<(POSM != 0), u64, u32>::type>;
+};
+template <>
+struct ListOf<2, 2> {
+    using type = Top2P;
+};
+
+// Leader lanes store subrange s's delegates, positions or packed offsets.
 // POSM: 0 none, 1 absolute positions (u32 per delegate), 2 packed offsets of the first
 // beta-1 delegates inside the subrange (8 bits each; u8 / u16 / u32 per subrange).
-// FUSE: 0 none, 1 radix top-digit histogram into lhist, 2 min/max into *mn/*mx.
-template <int BETA, int POSM, int FUSE, bool COMPOSITE>
-__device__ __forceinline__ void emit(const TopList<BETA>& t, u64 s, int alpha, u32* dvals, void* dpos,
-                                     u32* lhist, int hshift, u32* mn, u32* mx) {
+template <int BETA, int POSM, class LT>
+__device__ __forceinline__ void store_delegates(const LT& t, u64 s, int alpha, u32* dvals, void* dpos) {
     u32 packed = 0;
 #pragma unroll
     for (int j = 0; j < BETA; ++j) {
-        u32 val = COMPOSITE ? (u32)(t.v[j] >> 32) : (u32)t.v[j];
-        dvals[s * BETA + j] = val;
-        if (POSM != 0) {
-            u32 pos = t.v[j] == 0 ? 0xFFFFFFFFu : 0xFFFFFFFFu - (u32)t.v[j];
-            if (POSM == 1) ((u32*)dpos)[s * BETA + j] = pos;
-            if (POSM == 2 && j < BETA - 1) {
-                u32 off = pos == 0xFFFFFFFFu ? 0xFFu : (u32)(pos - (s << alpha)) & 0xFFu;
-                packed |= off << (8 * j);
-            }
+        dvals[s * BETA + j] = t.val(j);
+        if (POSM == 1) ((u32*)dpos)[s * BETA + j] = t.pos(j);
+        if (POSM == 2 && j < BETA - 1) {
+            const u32 p = t.pos(j);
+            const u32 off = p == 0xFFFFFFFFu ? 0xFFu : (u32)(p - (s << alpha)) & 0xFFu;
+            packed |= off << (8 * j);
         }
-        if (FUSE == 1) atomicAdd(&lhist[val >> hshift], 1u);
-        if (FUSE == 2) { *mn = min(*mn, val); *mx = max(*mx, val); }
     }
     if (POSM == 2) {
         if (BETA == 2) ((uint8_t*)dpos)[s] = (uint8_t)packed;
@@ -105,14 +151,33 @@ __device__ __forceinline__ void emit(const TopList<BETA>& t, u64 s, int alpha, u
     }
 }
 
+template <int BETA, int FUSE, class LT>
+__device__ __forceinline__ void fuse_add(const LT& t, bool leader, u32* lhist, int hshift, u32* mn,
+                                         u32* mx) {
+#pragma unroll
+    for (int j = 0; j < BETA; ++j) {
+        const u32 val = t.val(j);
+        if (FUSE == 1) wave_hist_add(lhist, leader, val >> hshift);
+        if (FUSE == 2 && leader) { *mn = min(*mn, val); *mx = max(*mx, val); }
+    }
+}
+
 template <int FUSE>
-__device__ __forceinline__ void fuse_flush(u32* lhist, u32* ghist, int bins, u32 mn, u32 mx, Ctrl* ctrl,
-                                           u32* red) {
+__device__ __forceinline__ void fuse_init(u32* lhist, int bins) {
+    if (FUSE == 1) {
+        for (int i = threadIdx.x; i < bins; i += blockDim.x) lhist[i] = 0;
+        __syncthreads();
+    }
+}
+
+template <int FUSE>
+__device__ __forceinline__ void fuse_flush(u32* lhist, u32* ghist, int bins, u32 mn, u32 mx, Ctrl* ctrl, u32* red) {
     if (FUSE == 1) {
         __syncthreads();
+        u32* g = ghist + (size_t)(blockIdx.x & (HIST_REPS - 1)) * bins;
         for (int i = threadIdx.x; i < bins; i += blockDim.x) {
             u32 c = lhist[i];
-            if (c) atomicAdd(&ghist[i], c);
+            if (c) atomicAdd(&g[i], c);
         }
     }
     if (FUSE == 2) {
@@ -123,7 +188,8 @@ __device__ __forceinline__ void fuse_flush(u32* lhist, u32* ghist, int bins, u32
         __syncthreads();
         if (threadIdx.x == 0) {
             for (int i = 1; i < (int)(blockDim.x >> 6); ++i) { mn = min(mn, red[i]); mx = max(mx, red[8 + i]); }
-            mn = min(mn, red[0]); mx = max(mx, red[8]);
+            mn = min(mn, red[0]);
+            mx = max(mx, red[8]);
             atomicMin(&ctrl->first.lo, mn);
             atomicMax(&ctrl->first.hi, mx);
         }
@@ -134,195 +200,266 @@ __device__ __forceinline__ u32x4 load4(const u32x4* p) { return __builtin_nontem
 
 // ---------------------------------------------------------------- lane groups, alpha 1..8
 constexpr int GROUP_BATCHES = 8;                      // 16-byte loads in flight per lane
-constexpr int GROUP_CHUNK = GROUP_BATCHES * WG * 4;   // 8192 keys per workgroup
+constexpr int GROUP_CHUNK = GROUP_BATCHES * WG * 4;   // 8192 keys per workgroup chunk
 
-template <class K, int BETA, int POSM, int FUSE>
+template <class K, int BETA, int POSM, int FUSE, class LT>
 __global__ __launch_bounds__(WG) void k_sample_groups(const u32* __restrict__ keys, u64 n, int alpha, int vec_ok,
-                                                      u32* __restrict__ dvals, void* __restrict__ dpos,
+                                                      u64 nchunks, u32* __restrict__ dvals, void* __restrict__ dpos,
                                                       u32* __restrict__ ghist, int hshift, int bins, Ctrl* ctrl) {
-    constexpr bool COMP = (BETA > 1) || (POSM != 0);
     __shared__ u32 lhist[FUSE == 1 ? 256 : 1];
     __shared__ u32 red[16];
-    if (FUSE == 1) {
-        for (int i = threadIdx.x; i < bins; i += WG) lhist[i] = 0;
-        __syncthreads();
-    }
+    fuse_init<FUSE>(lhist, bins);
     u32 mn = 0xFFFFFFFFu, mx = 0u;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const u64 c0 = (u64)blockIdx.x * GROUP_CHUNK;
     const u64 S = (n + (1ull << alpha) - 1) >> alpha;
-    u32x4 raw[GROUP_BATCHES];
-    const bool full = vec_ok && c0 + GROUP_CHUNK <= n;
-    const u32x4* base4 = (const u32x4*)(keys + c0);
+    for (u64 chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const u64 c0 = chunk * GROUP_CHUNK;
+        u32x4 raw[GROUP_BATCHES];
+        const bool full = vec_ok && c0 + GROUP_CHUNK <= n;
+        const u32x4* base4 = (const u32x4*)(keys + c0);
 #pragma unroll
-    for (int b = 0; b < GROUP_BATCHES; ++b) {
-        const int q = b * WG + w * 64 + lane;  // uint4 index inside the chunk
-        if (full) raw[b] = load4(base4 + q);
-        else {
-            u64 i0 = c0 + 4ull * q;
-            u32x4 r = {0, 0, 0, 0};
-            if (i0 + 0 < n) r.x = keys[i0 + 0];
-            if (i0 + 1 < n) r.y = keys[i0 + 1];
-            if (i0 + 2 < n) r.z = keys[i0 + 2];
-            if (i0 + 3 < n) r.w = keys[i0 + 3];
-            raw[b] = r;
-        }
-    }
-#pragma unroll
-    for (int b = 0; b < GROUP_BATCHES; ++b) {
-        const u64 i0 = c0 + 4ull * (b * WG + w * 64 + lane);
-        u32 kv[4] = {raw[b].x, raw[b].y, raw[b].z, raw[b].w};
-        if (alpha == 1) {
-            // two subranges of two keys in this lane
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                TopList<BETA> t;
-                t.clear();
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    u64 i = i0 + 2 * h + c;
-                    if (i < n) t.insert(elem<COMP>(K::to_ord(kv[2 * h + c]), i));
-                }
-                u64 s = (i0 >> 1) + h;
-                if (i0 + 2 * h < n)
-                    emit<BETA, POSM, FUSE, COMP>(t, s, alpha, dvals, dpos, lhist, hshift, &mn, &mx);
+        for (int b = 0; b < GROUP_BATCHES; ++b) {
+            const int q = b * WG + w * 64 + lane;  // uint4 index inside the chunk
+            if (full) raw[b] = load4(base4 + q);
+            else {
+                const u64 i0 = c0 + 4ull * q;
+                u32x4 r = {0, 0, 0, 0};
+                if (i0 + 0 < n) r.x = keys[i0 + 0];
+                if (i0 + 1 < n) r.y = keys[i0 + 1];
+                if (i0 + 2 < n) r.z = keys[i0 + 2];
+                if (i0 + 3 < n) r.w = keys[i0 + 3];
+                raw[b] = r;
             }
-        } else {
-            TopList<BETA> t;
-            t.clear();
+        }
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-                if (i0 + c < n) t.insert(elem<COMP>(K::to_ord(kv[c]), i0 + c));
-            const int g = 1 << (alpha - 2);  // lanes per subrange (1..64)
-            for (int off = g >> 1; off > 0; off >>= 1) t.shfl_merge(off);
-            const u64 s = i0 >> alpha;
-            if ((lane & (g - 1)) == 0 && s < S && i0 < n)
-                emit<BETA, POSM, FUSE, COMP>(t, s, alpha, dvals, dpos, lhist, hshift, &mn, &mx);
+        for (int b = 0; b < GROUP_BATCHES; ++b) {
+            const u64 i0 = c0 + 4ull * (b * WG + w * 64 + lane);
+            const u32 kv[4] = {K::to_ord(raw[b].x), K::to_ord(raw[b].y), K::to_ord(raw[b].z), K::to_ord(raw[b].w)};
+            if (alpha == 1) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    LT t;
+                    t.clear();
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+                        if (i0 + 2 * h + c < n) t.insert(kv[2 * h + c], i0 + 2 * h + c);
+                    const bool leader = i0 + 2 * h < n;
+                    if (leader) store_delegates<BETA, POSM, LT>(t, (i0 >> 1) + h, alpha, dvals, dpos);
+                    fuse_add<BETA, FUSE, LT>(t, leader, lhist, hshift, &mn, &mx);
+                }
+            } else {
+                LT t;
+                t.clear();
+                if (full) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) t.insert(kv[c], i0 + c);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        if (i0 + c < n) t.insert(kv[c], i0 + c);
+                }
+                const int g = 1 << (alpha - 2);  // lanes per subrange (1..64)
+                for (int off = g >> 1; off > 0; off >>= 1) t.shfl_merge(off);
+                const u64 s = i0 >> alpha;
+                const bool leader = (lane & (g - 1)) == 0 && i0 < n && s < S;
+                if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos);
+                fuse_add<BETA, FUSE, LT>(t, leader, lhist, hshift, &mn, &mx);
+            }
         }
     }
     fuse_flush<FUSE>(lhist, ghist, bins, mn, mx, ctrl, red);
 }
 
-// ---------------------------------------------------------------- wave per subrange, alpha >= 9
-template <class K, int BETA, int POSM, int FUSE>
-__global__ __launch_bounds__(WG) void k_sample_waves(const u32* __restrict__ keys, u64 n, int alpha, int vec_ok, int spw,
-                                                     u32* __restrict__ dvals, void* __restrict__ dpos,
+// ---------------------------------------------------------------- wave units, alpha >= 9
+template <class K, int BETA, int POSM, int FUSE, class LT>
+__global__ __launch_bounds__(WG) void k_sample_waves(const u32* __restrict__ keys, u64 n, int alpha, int vec_ok,
+                                                     u64 nunits, u32* __restrict__ dvals, void* __restrict__ dpos,
                                                      u32* __restrict__ ghist, int hshift, int bins, Ctrl* ctrl) {
-    constexpr bool COMP = (BETA > 1) || (POSM != 0);
     __shared__ u32 lhist[FUSE == 1 ? 256 : 1];
     __shared__ u32 red[16];
-    if (FUSE == 1) {
-        for (int i = threadIdx.x; i < bins; i += WG) lhist[i] = 0;
-        __syncthreads();
-    }
+    fuse_init<FUSE>(lhist, bins);
     u32 mn = 0xFFFFFFFFu, mx = 0u;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const u64 L = 1ull << alpha;
     const u64 S = (n + L - 1) >> alpha;
-    const int per_lane = (int)(L >> 8);  // 16-byte loads per lane per subrange (>= 2)
-    for (int r = 0; r < spw; ++r) {
-        const u64 s = ((u64)blockIdx.x * 4 + w) * spw + r;
-        if (s >= S) break;
-        const u64 sb = s << alpha;
-        TopList<BETA> t;
-        t.clear();
-        const bool full = vec_ok && sb + L <= n;
-        if (full) {
-            const u32x4* p = (const u32x4*)(keys + sb) + lane;
-            for (int j0 = 0; j0 < per_lane; j0 += 8) {
+    const int ps = (int)(L >> 8);            // 16-byte loads per lane per subrange (>= 2)
+    const u64 U = L > 2048 ? L : 2048;       // keys per wave unit
+    const int spu = (int)(U >> alpha);       // subranges per unit
+    const u64 nwaves = (u64)gridDim.x * (WG / WAVE);
+    for (u64 unit = (u64)blockIdx.x * (WG / WAVE) + w; unit < nunits; unit += nwaves) {
+        const u64 u0 = unit * U;
+        const bool full = vec_ok && u0 + U <= n;
+        if (full && spu > 1) {
+            // 8 loads per lane cover spu subranges of ps loads each
+            u32x4 raw[8];
+            const u32x4* p = (const u32x4*)(keys + u0) + lane;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) raw[j] = load4(p + j * 64);
+            for (int r = 0; r < spu; ++r) {
+                LT t;
+                t.clear();
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (j / ps != r) continue;
+                    const u64 i0 = u0 + 4ull * (j * 64 + lane);
+                    t.insert(K::to_ord(raw[j].x), i0);
+                    t.insert(K::to_ord(raw[j].y), i0 + 1);
+                    t.insert(K::to_ord(raw[j].z), i0 + 2);
+                    t.insert(K::to_ord(raw[j].w), i0 + 3);
+                }
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) t.shfl_merge(off);
+                const u64 s = (u0 >> alpha) + r;
+                if (lane == 0) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos);
+                fuse_add<BETA, FUSE, LT>(t, lane == 0, lhist, hshift, &mn, &mx);
+            }
+        } else if (full) {
+            // one subrange per unit, ps >= 8 loads per lane in batches of 8
+            LT t;
+            t.clear();
+            const u32x4* p = (const u32x4*)(keys + u0) + lane;
+            for (int j0 = 0; j0 < ps; j0 += 8) {
                 u32x4 raw[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (j0 + u < per_lane) raw[u] = load4(p + (j0 + u) * 64);
+                for (int u = 0; u < 8; ++u) raw[u] = load4(p + (j0 + u) * 64);
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (j0 + u < per_lane) {
-                        u64 i0 = sb + 4ull * ((j0 + u) * 64 + lane);
-                        if (BETA == 1 && !COMP) {
-                            u32 m = max(max(K::to_ord(raw[u].x), K::to_ord(raw[u].y)),
-                                        max(K::to_ord(raw[u].z), K::to_ord(raw[u].w)));
-                            t.v[0] = t.v[0] > m ? t.v[0] : m;
-                        } else {
-                            t.insert(elem<COMP>(K::to_ord(raw[u].x), i0));
-                            t.insert(elem<COMP>(K::to_ord(raw[u].y), i0 + 1));
-                            t.insert(elem<COMP>(K::to_ord(raw[u].z), i0 + 2));
-                            t.insert(elem<COMP>(K::to_ord(raw[u].w), i0 + 3));
-                        }
+                for (int u = 0; u < 8; ++u) {
+                    const u64 i0 = u0 + 4ull * ((j0 + u) * 64 + lane);
+                    if (BETA == 1 && POSM == 0) {
+                        const u32 m = max(max(K::to_ord(raw[u].x), K::to_ord(raw[u].y)),
+                                          max(K::to_ord(raw[u].z), K::to_ord(raw[u].w)));
+                        t.insert(m, 0);
+                    } else {
+                        t.insert(K::to_ord(raw[u].x), i0);
+                        t.insert(K::to_ord(raw[u].y), i0 + 1);
+                        t.insert(K::to_ord(raw[u].z), i0 + 2);
+                        t.insert(K::to_ord(raw[u].w), i0 + 3);
                     }
+                }
             }
-        } else {
-            for (u64 i = sb + lane; i < sb + L && i < n; i += 64) t.insert(elem<COMP>(K::to_ord(keys[i]), i));
-        }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) t.shfl_merge(off);
-        if (lane == 0) emit<BETA, POSM, FUSE, COMP>(t, s, alpha, dvals, dpos, lhist, hshift, &mn, &mx);
+            for (int off = 32; off > 0; off >>= 1) t.shfl_merge(off);
+            const u64 s = u0 >> alpha;
+            if (lane == 0) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos);
+            fuse_add<BETA, FUSE, LT>(t, lane == 0, lhist, hshift, &mn, &mx);
+        } else {
+            // tail unit: scalar loads, one subrange at a time
+            for (int r = 0; r < spu; ++r) {
+                const u64 sb = u0 + (u64)r * L;
+                if (sb >= n) break;
+                LT t;
+                t.clear();
+                for (u64 i = sb + lane; i < sb + L && i < n; i += 64) t.insert(K::to_ord(keys[i]), i);
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) t.shfl_merge(off);
+                const u64 s = sb >> alpha;
+                const bool leader = lane == 0 && s < S;
+                if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos);
+                fuse_add<BETA, FUSE, LT>(t, leader, lhist, hshift, &mn, &mx);
+            }
+        }
     }
     fuse_flush<FUSE>(lhist, ghist, bins, mn, mx, ctrl, red);
 }
 
 // ---------------------------------------------------------------- LDS staged, alpha 1..5
-template <class K, int BETA, int POSM, int FUSE>
-__global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ keys, u64 n, int alpha,
-                                                      u32* __restrict__ dvals, void* __restrict__ dpos,
+template <class K, int BETA, int POSM, int FUSE, class LT>
+__global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ keys, u64 n, int alpha, int vec_ok,
+                                                      u64 ntiles, u32* __restrict__ dvals, void* __restrict__ dpos,
                                                       u32* __restrict__ ghist, int hshift, int bins, Ctrl* ctrl) {
-    constexpr bool COMP = (BETA > 1) || (POSM != 0);
-    // 256 subranges of up to 32 keys; one pad word after every 32 keys breaks bank conflicts
-    __shared__ u32 tile[WG * 32 + WG];
+    // 256 subranges of up to 32 keys; one pad uint4 after every 8 uint4 spreads LDS banks
+    __shared__ u32x4 tile[WG * 8 + WG];
     __shared__ u32 lhist[FUSE == 1 ? 256 : 1];
     __shared__ u32 red[16];
-    if (FUSE == 1) {
-        for (int i = threadIdx.x; i < bins; i += WG) lhist[i] = 0;
-    }
+    fuse_init<FUSE>(lhist, bins);
     u32 mn = 0xFFFFFFFFu, mx = 0u;
     const u64 L = 1ull << alpha;
     const u64 S = (n + L - 1) >> alpha;
-    const u64 s0 = (u64)blockIdx.x * WG;
-    const u64 k0 = s0 << alpha;
-    const int tile_keys = WG << alpha;
-    // coalesced staging: key j of the tile goes to tile[j + j/32]
-    for (int j = threadIdx.x; j < tile_keys; j += WG) {
-        u64 i = k0 + j;
-        u32 v = i < n ? K::to_ord(keys[i]) : 0u;
-        tile[j + (j >> 5)] = v;
-    }
-    __syncthreads();
-    const u64 s = s0 + threadIdx.x;
-    if (s < S) {
-        TopList<BETA> t;
-        t.clear();
-        const int jb = threadIdx.x << alpha;
-        for (int c = 0; c < (int)L; ++c) {
-            u64 i = k0 + jb + c;
-            if (i < n) {
-                int j = jb + c;
-                t.insert(elem<COMP>(tile[j + (j >> 5)], i));
+    const int qpt = alpha >= 2 ? (int)(L >> 2) : 1;  // uint4 per thread
+    for (u64 tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
+        const u64 s0 = tt * WG;
+        const u64 k0 = s0 << alpha;
+        const int tile_q = (WG << alpha) >> 2;  // uint4 in the tile
+        const bool full = vec_ok && k0 + ((u64)WG << alpha) <= n && alpha >= 2;
+        if (full) {
+            const u32x4* p = (const u32x4*)(keys + k0);
+            for (int q = threadIdx.x; q < tile_q; q += WG) {
+                u32x4 r = load4(p + q);
+                r.x = K::to_ord(r.x); r.y = K::to_ord(r.y); r.z = K::to_ord(r.z); r.w = K::to_ord(r.w);
+                tile[q + (q >> 3)] = r;
+            }
+        } else {
+            for (int q = threadIdx.x; q < ((WG << alpha) + 3) / 4; q += WG) {
+                u32x4 r = {0, 0, 0, 0};
+                const u64 i = k0 + 4ull * q;
+                if (i + 0 < n) r.x = K::to_ord(keys[i + 0]);
+                if (i + 1 < n) r.y = K::to_ord(keys[i + 1]);
+                if (i + 2 < n) r.z = K::to_ord(keys[i + 2]);
+                if (i + 3 < n) r.w = K::to_ord(keys[i + 3]);
+                tile[q + (q >> 3)] = r;
             }
         }
-        emit<BETA, POSM, FUSE, COMP>(t, s, alpha, dvals, dpos, lhist, hshift, &mn, &mx);
+        __syncthreads();
+        const u64 s = s0 + threadIdx.x;
+        LT t;
+        t.clear();
+        const u64 kb = k0 + ((u64)threadIdx.x << alpha);
+        if (alpha >= 2) {
+            for (int i = 0; i < qpt; ++i) {
+                const int q = threadIdx.x * qpt + i;
+                const u32x4 r = tile[q + (q >> 3)];
+                const u64 i0 = kb + 4ull * i;
+                if (i0 + 3 < n) {
+                    t.insert(r.x, i0); t.insert(r.y, i0 + 1);
+                    t.insert(r.z, i0 + 2); t.insert(r.w, i0 + 3);
+                } else {
+                    if (i0 + 0 < n) t.insert(r.x, i0);
+                    if (i0 + 1 < n) t.insert(r.y, i0 + 1);
+                    if (i0 + 2 < n) t.insert(r.z, i0 + 2);
+                }
+            }
+        } else {
+            // alpha == 1: two keys per subrange
+            const int j = threadIdx.x * 2;
+            const int q = j >> 2;
+            const u32x4 r = tile[q + (q >> 3)];
+            const u32 a = (j & 3) == 0 ? r.x : r.z, b = (j & 3) == 0 ? r.y : r.w;
+            if (kb < n) t.insert(a, kb);
+            if (kb + 1 < n) t.insert(b, kb + 1);
+        }
+        const bool leader = s < S;
+        if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos);
+        fuse_add<BETA, FUSE, LT>(t, leader, lhist, hshift, &mn, &mx);
+        __syncthreads();
     }
     fuse_flush<FUSE>(lhist, ghist, bins, mn, mx, ctrl, red);
 }
 
 // ---------------------------------------------------------------- dispatch
+constexpr u64 SAMPLE_GRID = 8192;  // persistent workgroups (32 per CU)
+
 template <class K, int BETA, int POSM, int FUSE>
 static hipError_t launch_sample_t(const SampleArgs& a, hipStream_t st) {
+    using LT = typename ListOf<BETA, POSM>::type;
     const u64 L = 1ull << a.alpha;
     const u64 S = (a.n + L - 1) >> a.alpha;
     const int vec_ok = ((uintptr_t)a.keys & 15) == 0;
     if (a.staged && a.alpha <= 5) {
-        unsigned grid = (unsigned)((S + WG - 1) / WG);
-        hipLaunchKernelGGL((k_sample_staged<K, BETA, POSM, FUSE>), dim3(grid), dim3(WG), 0, st, a.keys, a.n,
-                           a.alpha, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl);
+        const u64 ntiles = (S + WG - 1) / WG;
+        const unsigned grid = (unsigned)std::min<u64>(ntiles, SAMPLE_GRID);
+        hipLaunchKernelGGL((k_sample_staged<K, BETA, POSM, FUSE, LT>), dim3(grid), dim3(WG), 0, st, a.keys, a.n,
+                           a.alpha, vec_ok, ntiles, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl);
     } else if (a.alpha <= 8) {
-        unsigned grid = (unsigned)((a.n + GROUP_CHUNK - 1) / GROUP_CHUNK);
-        hipLaunchKernelGGL((k_sample_groups<K, BETA, POSM, FUSE>), dim3(grid), dim3(WG), 0, st, a.keys, a.n,
-                           a.alpha, vec_ok, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl);
+        const u64 nchunks = (a.n + GROUP_CHUNK - 1) / GROUP_CHUNK;
+        const unsigned grid = (unsigned)std::min<u64>(nchunks, SAMPLE_GRID);
+        hipLaunchKernelGGL((k_sample_groups<K, BETA, POSM, FUSE, LT>), dim3(grid), dim3(WG), 0, st, a.keys, a.n,
+                           a.alpha, vec_ok, nchunks, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl);
     } else {
-        int spw = a.alpha >= 11 ? 1 : (2048 >> a.alpha);
-        u64 waves = (S + spw - 1) / spw;
-        unsigned grid = (unsigned)((waves + 3) / 4);
-        hipLaunchKernelGGL((k_sample_waves<K, BETA, POSM, FUSE>), dim3(grid), dim3(WG), 0, st, a.keys, a.n,
-                           a.alpha, vec_ok, spw, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl);
+        const u64 U = L > 2048 ? L : 2048;
+        const u64 nunits = (a.n + U - 1) / U;
+        const unsigned grid = (unsigned)std::min<u64>((nunits + 3) / 4, SAMPLE_GRID);
+        hipLaunchKernelGGL((k_sample_waves<K, BETA, POSM, FUSE, LT>), dim3(grid), dim3(WG), 0, st, a.keys, a.n,
+                           a.alpha, vec_ok, nunits, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl);
     }
     return hipGetLastError();
 }

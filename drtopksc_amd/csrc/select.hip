@@ -1,6 +1,6 @@
 // select.hip -- the device-resident selection machinery of Dr.Top-k.
 //
-//  k_init        writes the control block and clears histograms / look-back state
+//  k_init        writes the control block and clears histograms and ticket counters
 //                (a kernel, not hipMemsetAsync, so the pipeline can be graph-captured).
 //  k_hist        one histogram pass: radix digit of keys matching the current prefix
 //                (CalculateOccurence_inplace[_firsttopk], radixselect.cuh:353-410) or
@@ -9,19 +9,8 @@
 //  k_select      one workgroup: cumulative counts, the k-digit / k-bucket, state update
 //                (CumulateCount_inplace radixselect.cuh:192-220 and the host loop
 //                :1427-1445; FindKBucket bucketselect_combined.cuh:200-216).  No D2H copy.
-//  k_range_default  FindTopKRanges + FillWithKElementsID (radixselect.cuh:413-427,
-//                :603-618) as one ordered compaction: delegates above the threshold, then
-//                the lowest-id delegates equal to it, exactly k ids in ascending order.
-//  k_gather_default CreateConcatenateRange[_optimized] (radixselect.cuh:1546-1607): the
-//                keys (>= threshold with the filter) of the selected subranges, with their
-//                indices, in index order.
-//  k_range_gather_beta FindTopKRanges_beta + CreateConcatenateRange_Beta_optimized
-//                (radixselect.cuh:472-600, :1610-1639) fused: per subrange, all beta
-//                delegates >= threshold -> its keys, else if the maximum >= threshold ->
-//                its delegates ("lonely").
-//  k_extract     findTopk + FillTopk (MultiGPU/radixselect.cuh:1355-1383): keys above the
-//                k-th largest plus the lowest-index keys equal to it, in index order.
-// Ordered compactions use the decoupled look-back of common.hpp over 4096-element tiles.
+//  *_seg         the same passes over a segmented concatenation (compact.hip).
+// Range selection, concatenation and top-k extraction live in compact.hip.
 #include <algorithm>
 
 #include "common.hpp"
@@ -29,8 +18,6 @@
 
 namespace drtopk {
 
-constexpr int TILE_E = 16;              // elements per thread per tile
-constexpr int TILE = WG * TILE_E;       // 4096 elements per tile
 constexpr int MAX_LDS_BINS = 4096;
 
 // ---------------------------------------------------------------- init
@@ -64,12 +51,42 @@ __device__ __forceinline__ u32 bucket_hi(u64 b, u32 lo, u32 hi, u64 B) {
     return b + 1 == B ? hi : bucket_lo(b + 1, lo, hi, B) - 1;
 }
 
+// ---------------------------------------------------------------- element iteration
+// Calls f(v) for this workgroup's share of a flat array (grid-stride, 16-byte loads) or of
+// a segmented array (one segment per workgroup at a time).
+template <class F>
+__device__ __forceinline__ void for_flat(const u32* arr, u64 m, F&& f) {
+    const u64 nv = m >> 2;
+    const u32x4* a4 = (const u32x4*)arr;
+    const u64 stride = (u64)gridDim.x * WG;
+    for (u64 i = (u64)blockIdx.x * WG + threadIdx.x; i < nv; i += stride) {
+        const u32x4 r = a4[i];
+        f(r.x); f(r.y); f(r.z); f(r.w);
+    }
+    if (blockIdx.x == 0)
+        for (u64 i = (nv << 2) + threadIdx.x; i < m; i += WG) f(arr[i]);
+}
+template <class F>
+__device__ __forceinline__ void for_seg(const Seg& g, F&& f) {
+    for (u64 s = blockIdx.x; s < g.nseg; s += gridDim.x) {
+        const u64 len = g.cnt ? (u64)g.cnt[s] : std::min<u64>(g.segcap, g.n - s * g.segcap);
+        const u32* base = g.keys + s * g.segcap;  // segcap is a multiple of 4: 16-byte aligned
+        const u64 nv = len >> 2;
+        const u32x4* a4 = (const u32x4*)base;
+        for (u64 i = threadIdx.x; i < nv; i += WG) {
+            const u32x4 r = a4[i];
+            f(r.x); f(r.y); f(r.z); f(r.w);
+        }
+        for (u64 i = (nv << 2) + threadIdx.x; i < len; i += WG) f(base[i]);
+    }
+}
+
 // ---------------------------------------------------------------- histogram pass
-template <int MODE, bool GLOBALH, class K, bool RAW>
-__global__ __launch_bounds__(WG) void k_hist(const u32* __restrict__ arr, SelState* sel, u32* ghist, int bins) {
+template <int MODE, bool GLOBALH, class K, bool RAW, bool SEG>
+__global__ __launch_bounds__(WG) void k_hist(const u32* __restrict__ arr, Seg seg, SelState* sel, u32* ghist,
+                                             int bins) {
     __shared__ u32 lh[GLOBALH ? 1 : MAX_LDS_BINS];
     if (sel->done) return;
-    const u64 m = sel->m;
     u32 flag = 0, mask = 0, lo = 0, W = 0, Bm1 = 0;
     int shright = 0;
     double inv = 0.0;
@@ -99,41 +116,42 @@ __global__ __launch_bounds__(WG) void k_hist(const u32* __restrict__ arr, SelSta
             if (v >= lo && v - lo <= W) atomicAdd(&h[bucket_of(v, lo, W, Bm1, inv)], 1u);
         }
     };
-    const u64 nv = m >> 2;
-    const u32x4* a4 = (const u32x4*)arr;
-    const u64 stride = (u64)gridDim.x * WG;
-    for (u64 i = (u64)blockIdx.x * WG + threadIdx.x; i < nv; i += stride) {
-        u32x4 r = a4[i];
-        count(r.x); count(r.y); count(r.z); count(r.w);
-    }
-    if (blockIdx.x == 0)
-        for (u64 i = (nv << 2) + threadIdx.x; i < m; i += WG) count(arr[i]);
+    if (SEG) for_seg(seg, count);
+    else for_flat(arr, sel->m, count);
     if (!GLOBALH) {
         __syncthreads();
+        u32* g = ghist + (size_t)(blockIdx.x % hist_reps(bins)) * bins;
         for (int i = threadIdx.x; i < bins; i += WG) {
             u32 c = lh[i];
-            if (c) atomicAdd(&ghist[i], c);
+            if (c) atomicAdd(&g[i], c);
         }
     }
 }
 
-template <class K, bool RAW>
-static hipError_t launch_hist_t(int mode, const u32* arr, SelState* sel, u32* ghist, int bins, unsigned grid,
-                                hipStream_t st) {
+template <class K, bool RAW, bool SEG>
+static hipError_t launch_hist_t(int mode, const u32* arr, const Seg& seg, SelState* sel, u32* ghist, int bins,
+                                unsigned grid, hipStream_t st) {
     const bool g = bins > MAX_LDS_BINS;
-    if (mode == 0 && !g) hipLaunchKernelGGL((k_hist<0, false, K, RAW>), dim3(grid), dim3(WG), 0, st, arr, sel, ghist, bins);
-    else if (mode == 0) hipLaunchKernelGGL((k_hist<0, true, K, RAW>), dim3(grid), dim3(WG), 0, st, arr, sel, ghist, bins);
-    else if (!g) hipLaunchKernelGGL((k_hist<1, false, K, RAW>), dim3(grid), dim3(WG), 0, st, arr, sel, ghist, bins);
+    if (mode == 0 && !g)
+        hipLaunchKernelGGL((k_hist<0, false, K, RAW, SEG>), dim3(grid), dim3(WG), 0, st, arr, seg, sel, ghist, bins);
+    else if (mode == 0)
+        hipLaunchKernelGGL((k_hist<0, true, K, RAW, SEG>), dim3(grid), dim3(WG), 0, st, arr, seg, sel, ghist, bins);
+    else if (!g)
+        hipLaunchKernelGGL((k_hist<1, false, K, RAW, SEG>), dim3(grid), dim3(WG), 0, st, arr, seg, sel, ghist, bins);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 hipError_t launch_hist(int mode, const u32* arr, SelState* sel, u32* ghist, int bins, unsigned grid, hipStream_t st) {
-    return launch_hist_t<KeyU32, false>(mode, arr, sel, ghist, bins, grid, st);
+    return launch_hist_t<KeyU32, false, false>(mode, arr, Seg{}, sel, ghist, bins, grid, st);
 }
 hipError_t launch_hist_raw(int mode, int f32, const u32* keys, SelState* sel, u32* ghist, int bins, unsigned grid,
                            hipStream_t st) {
-    return f32 ? launch_hist_t<KeyF32, true>(mode, keys, sel, ghist, bins, grid, st)
-               : launch_hist_t<KeyU32, true>(mode, keys, sel, ghist, bins, grid, st);
+    return f32 ? launch_hist_t<KeyF32, true, false>(mode, keys, Seg{}, sel, ghist, bins, grid, st)
+               : launch_hist_t<KeyU32, true, false>(mode, keys, Seg{}, sel, ghist, bins, grid, st);
+}
+hipError_t launch_hist_seg(int mode, const Seg& seg, SelState* sel, u32* ghist, int bins, unsigned grid,
+                           hipStream_t st) {
+    return launch_hist_t<KeyU32, false, true>(mode, nullptr, seg, sel, ghist, bins, grid, st);
 }
 
 // ---------------------------------------------------------------- select step
@@ -148,8 +166,14 @@ __global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState
     if (sel->done) return;
     const int per = (bins + SEL_T - 1) / SEL_T;
     const int b0 = min(bins, (int)threadIdx.x * per), b1 = min(bins, b0 + per);
+    const int reps = hist_reps(bins);
+    auto bin_count = [&](int b) -> u64 {
+        u64 c = 0;
+        for (int r = 0; r < reps; ++r) c += ghist[(size_t)r * bins + b];
+        return c;
+    };
     u64 sum = 0;
-    for (int b = b0; b < b1; ++b) sum += ghist[b];
+    for (int b = b0; b < b1; ++b) sum += bin_count(b);
     u64 tot;
     const u64 excl = block_excl_scan_u64<SEL_T>(sum, scratch, &tot);
     const u64 m = sel->m, k = sel->k, below = sel->below;
@@ -159,13 +183,14 @@ __global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState
     if (excl < T && excl + sum >= T) {
         u64 run = excl;
         for (int b = b0; b < b1; ++b) {
-            u64 c = ghist[b];
+            u64 c = bin_count(b);
             if (run + c >= T) { found[0] = (u64)b; found[1] = run; found[2] = c; break; }
             run += c;
         }
     }
     __syncthreads();
-    for (int b = b0; b < b1; ++b) ghist[b] = 0;  // ready for the next pass
+    for (int r = 0; r < reps; ++r)  // ready for the next pass
+        for (int b = b0; b < b1; ++b) ghist[(size_t)r * bins + b] = 0;
     if (threadIdx.x != 0) return;
     if (found[0] == ~0ull) {
         atomicOr(&ctrl->error, ERR_CONCAT_SHORT);
@@ -214,17 +239,17 @@ hipError_t launch_select(int mode, u32* ghist, int bins, SelState* sel, Ctrl* ct
 
 // ---------------------------------------------------------------- bucket range set-up
 // min/max over an array (second top-k), into sel->lo / sel->hi (initialised to ~0 / 0).
-template <class K, bool RAW>
-__global__ __launch_bounds__(WG) void k_minmax(const u32* __restrict__ arr, SelState* sel) {
+template <class K, bool RAW, bool SEG>
+__global__ __launch_bounds__(WG) void k_minmax(const u32* __restrict__ arr, Seg seg, SelState* sel) {
     __shared__ u32 red[16];
-    const u64 m = sel->m;
     u32 mn = 0xFFFFFFFFu, mx = 0;
-    const u64 stride = (u64)gridDim.x * WG;
-    for (u64 i = (u64)blockIdx.x * WG + threadIdx.x; i < m; i += stride) {
-        u32 v = RAW ? K::to_ord(arr[i]) : arr[i];
+    auto f = [&](u32 raw) {
+        const u32 v = RAW ? K::to_ord(raw) : raw;
         mn = min(mn, v);
         mx = max(mx, v);
-    }
+    };
+    if (SEG) for_seg(seg, f);
+    else for_flat(arr, sel->m, f);
     mn = wave_min_u32(mn);
     mx = wave_max_u32(mx);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -239,12 +264,16 @@ __global__ __launch_bounds__(WG) void k_minmax(const u32* __restrict__ arr, SelS
     }
 }
 hipError_t launch_minmax(const u32* arr, SelState* sel, Ctrl*, unsigned grid, hipStream_t st) {
-    hipLaunchKernelGGL((k_minmax<KeyU32, false>), dim3(grid), dim3(WG), 0, st, arr, sel);
+    hipLaunchKernelGGL((k_minmax<KeyU32, false, false>), dim3(grid), dim3(WG), 0, st, arr, Seg{}, sel);
     return hipGetLastError();
 }
 hipError_t launch_minmax_raw(int f32, const u32* keys, SelState* sel, Ctrl*, unsigned grid, hipStream_t st) {
-    if (f32) hipLaunchKernelGGL((k_minmax<KeyF32, true>), dim3(grid), dim3(WG), 0, st, keys, sel);
-    else hipLaunchKernelGGL((k_minmax<KeyU32, true>), dim3(grid), dim3(WG), 0, st, keys, sel);
+    if (f32) hipLaunchKernelGGL((k_minmax<KeyF32, true, false>), dim3(grid), dim3(WG), 0, st, keys, Seg{}, sel);
+    else hipLaunchKernelGGL((k_minmax<KeyU32, true, false>), dim3(grid), dim3(WG), 0, st, keys, Seg{}, sel);
+    return hipGetLastError();
+}
+hipError_t launch_minmax_seg(const Seg& seg, SelState* sel, unsigned grid, hipStream_t st) {
+    hipLaunchKernelGGL((k_minmax<KeyU32, false, true>), dim3(grid), dim3(WG), 0, st, nullptr, seg, sel);
     return hipGetLastError();
 }
 
@@ -266,452 +295,24 @@ hipError_t launch_bucket_begin(SelState* sel, Ctrl*, int, hipStream_t st) {
 
 // The k-bucket holds exactly one key: fetch it (GetKValue, bucketselect_combined.cuh:271-298;
 // unique, so there is a single writer).
-template <class K, bool RAW>
-__global__ __launch_bounds__(WG) void k_find(const u32* __restrict__ arr, SelState* sel) {
+template <class K, bool RAW, bool SEG>
+__global__ __launch_bounds__(WG) void k_find(const u32* __restrict__ arr, Seg seg, SelState* sel) {
     if (!sel->need_find) return;
-    const u64 m = sel->m;
     const u32 lo = sel->lo, hi = sel->hi;
-    const u64 stride = (u64)gridDim.x * WG;
-    for (u64 i = (u64)blockIdx.x * WG + threadIdx.x; i < m; i += stride) {
-        u32 v = RAW ? K::to_ord(arr[i]) : arr[i];
+    auto f = [&](u32 raw) {
+        const u32 v = RAW ? K::to_ord(raw) : raw;
         if (v >= lo && v <= hi) sel->result = v;
-    }
+    };
+    if (SEG) for_seg(seg, f);
+    else for_flat(arr, sel->m, f);
 }
 hipError_t launch_find(const u32* arr, int raw_f32, SelState* sel, unsigned grid, hipStream_t st) {
-    if (raw_f32 == 1) hipLaunchKernelGGL((k_find<KeyF32, true>), dim3(grid), dim3(WG), 0, st, arr, sel);
-    else hipLaunchKernelGGL((k_find<KeyU32, false>), dim3(grid), dim3(WG), 0, st, arr, sel);
+    if (raw_f32 == 1) hipLaunchKernelGGL((k_find<KeyF32, true, false>), dim3(grid), dim3(WG), 0, st, arr, Seg{}, sel);
+    else hipLaunchKernelGGL((k_find<KeyU32, false, false>), dim3(grid), dim3(WG), 0, st, arr, Seg{}, sel);
     return hipGetLastError();
 }
-
-// ---------------------------------------------------------------- range selection (default path)
-__global__ __launch_bounds__(WG) void k_range_default(const u32* __restrict__ dvals, u64 S, u64 k, int fullcmp,
-                                                      SelState* sel, Ctrl* ctrl, u32* __restrict__ sel_ids,
-                                                      u64* status, u32* ticket) {
-    __shared__ u64 scratch[WG / WAVE + 1];
-    __shared__ u64 tile_excl;
-    __shared__ u32 tslot;
-    const u32 thr = sel->result;
-    const int shleft = fullcmp ? 0 : (sel->digit + 1) * (int)sel->nb;
-    const u32 mask = shleft >= 32 ? 0u : ~((1u << shleft) - 1u);
-    const u64 total_gt = sel->m - sel->below - sel->kcount;
-    const u64 r1 = k - total_gt;
-    const u64 ntiles = (S + TILE - 1) / TILE;
-    for (;;) {
-        const u32 t = next_tile(ticket, &tslot);
-        if (t >= ntiles) break;
-        const u64 e0 = (u64)t * TILE + (u64)threadIdx.x * TILE_E;
-        u32 v[TILE_E];
-        u32 gt = 0, eq = 0;
-#pragma unroll
-        for (int j = 0; j < TILE_E; ++j) {
-            v[j] = e0 + j < S ? (dvals[e0 + j] & mask) : 0u;
-            bool in = e0 + j < S;
-            gt += in && v[j] > thr;
-            eq += in && v[j] == thr;
-        }
-        u64 tot;
-        const u64 ex = block_excl_scan_u64<WG>(pack2(gt, eq), scratch, &tot);
-        if (threadIdx.x == 0) tile_excl = lookback(status, t, tot, &ctrl->error);
-        __syncthreads();
-        const u64 te = tile_excl;
-        u64 g = unpack_hi(te) + unpack_hi(ex), q = unpack_lo(te) + unpack_lo(ex);
-#pragma unroll
-        for (int j = 0; j < TILE_E; ++j) {
-            if (e0 + j >= S) break;
-            u64 pos = ~0ull;
-            if (v[j] > thr) {
-                pos = g + (q < r1 ? q : r1);
-                ++g;
-            } else if (v[j] == thr) {
-                if (q < r1) pos = g + q;
-                ++q;
-            }
-            if (pos != ~0ull) {
-                if (pos < k) sel_ids[pos] = (u32)(e0 + j);
-                else atomicOr(&ctrl->error, ERR_CAPACITY);
-            }
-        }
-        __syncthreads();
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->selected = k;
-}
-
-hipError_t launch_range_default(const RangeArgs& a, hipStream_t st) {
-    const u64 ntiles = (a.S + TILE - 1) / TILE;
-    unsigned grid = (unsigned)std::min<u64>(ntiles, 2048);
-    hipLaunchKernelGGL(k_range_default, dim3(grid), dim3(WG), 0, st, a.dvals, a.S, a.k, a.mode, a.sel, a.ctrl,
-                       a.sel_ids, a.status, a.ticket);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------- gather (default path)
-template <class K, bool HIST2>
-__global__ __launch_bounds__(WG) void k_gather_default(const u32* __restrict__ keys, u64 n, int alpha, int filter,
-                                                       int vec_ok, const u32* __restrict__ sel_ids, u64 nsel,
-                                                       Ctrl* ctrl, u32* __restrict__ ckeys, u32* __restrict__ cidx,
-                                                       u64 cap, u64* status, u32* ticket, u32* ghist2, int hshift2,
-                                                       int bins2) {
-    __shared__ u64 scratch[WG / WAVE + 1];
-    __shared__ u64 tile_excl;
-    __shared__ u32 tslot;
-    __shared__ u32 lh[HIST2 ? 256 : 1];
-    if (HIST2) {
-        for (int i = threadIdx.x; i < bins2; i += WG) lh[i] = 0;
-    }
-    const u32 thr = ctrl->first.result;
-    const u64 L = 1ull << alpha, Lm = L - 1;
-    const u64 V = nsel << alpha;
-    const u64 ntiles = (V + TILE - 1) / TILE;
-    for (;;) {
-        const u32 t = next_tile(ticket, &tslot);
-        if (t >= ntiles) break;
-        const u64 v0 = (u64)t * TILE + (u64)threadIdx.x * TILE_E;
-        u32 kv[TILE_E];
-        u32 gi[TILE_E];
-        u32 cnt = 0;
-#pragma unroll
-        for (int q = 0; q < TILE_E / 4; ++q) {
-            const u64 vp = v0 + 4 * q;
-            u32 r[4] = {0, 0, 0, 0};
-            u64 g0 = 0;
-            bool any = vp < V;
-            if (any) {
-                if (L >= 4) {
-                    const u64 sid = sel_ids[vp >> alpha];
-                    g0 = (sid << alpha) + (vp & Lm);
-                    if (vec_ok && g0 + 3 < n) {
-                        u32x4 x = *(const u32x4*)(keys + g0);
-                        r[0] = x.x; r[1] = x.y; r[2] = x.z; r[3] = x.w;
-                    } else {
-#pragma unroll
-                        for (int c = 0; c < 4; ++c)
-                            if (g0 + c < n) r[c] = keys[g0 + c];
-                    }
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const u64 vpc = vp + c;
-                u64 g = g0 + c;
-                bool ok = vpc < V;
-                if (ok && L < 4) {
-                    const u64 sid = sel_ids[vpc >> alpha];
-                    g = (sid << alpha) + (vpc & Lm);
-                    if (g < n) r[c] = keys[g];
-                }
-                ok = ok && g < n;
-                const u32 o = K::to_ord(r[c]);
-                const bool take = ok && (!filter || o >= thr);
-                kv[4 * q + c] = take ? o : 0u;
-                gi[4 * q + c] = take ? (u32)g : 0xFFFFFFFFu;
-                cnt += take;
-            }
-        }
-        u64 tot;
-        const u64 ex = block_excl_scan_u64<WG>((u64)cnt, scratch, &tot);
-        if (threadIdx.x == 0) tile_excl = lookback(status, t, tot, &ctrl->error);
-        __syncthreads();
-        u64 pos = tile_excl + ex;
-#pragma unroll
-        for (int j = 0; j < TILE_E; ++j) {
-            if (gi[j] != 0xFFFFFFFFu) {
-                if (pos < cap) {
-                    ckeys[pos] = kv[j];
-                    if (cidx) cidx[pos] = gi[j];
-                } else {
-                    atomicOr(&ctrl->error, ERR_CAPACITY);
-                }
-                if (HIST2) atomicAdd(&lh[kv[j] >> hshift2], 1u);
-                ++pos;
-            }
-        }
-        if (t == ntiles - 1 && threadIdx.x == 0) {
-            u64 C = tile_excl + tot;
-            ctrl->concat_size = C;
-            if (C > cap) {
-                atomicOr(&ctrl->error, ERR_CAPACITY);
-                C = cap;
-            }
-            ctrl->second.m = C;
-            if (C < ctrl->second.k) atomicOr(&ctrl->error, ERR_CONCAT_SHORT);
-        }
-        __syncthreads();
-    }
-    if (HIST2) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < bins2; i += WG) {
-            u32 c = lh[i];
-            if (c) atomicAdd(&ghist2[i], c);
-        }
-    }
-}
-
-hipError_t launch_gather_default(const GatherArgs& a, hipStream_t st) {
-    const u64 V = a.nsel << a.alpha;
-    const u64 ntiles = (V + TILE - 1) / TILE;
-    unsigned grid = (unsigned)std::min<u64>(ntiles, 4096);
-    const int vec_ok = ((uintptr_t)a.keys & 15) == 0;
-#define GD_ARGS a.keys, a.n, a.alpha, a.filter, vec_ok, a.sel_ids, a.nsel, a.ctrl, a.ckeys, a.cidx, a.cap, a.status, \
-                a.ticket, a.ghist2, a.hshift2, a.bins2
-    if (a.f32) {
-        if (a.ghist2) hipLaunchKernelGGL((k_gather_default<KeyF32, true>), dim3(grid), dim3(WG), 0, st, GD_ARGS);
-        else hipLaunchKernelGGL((k_gather_default<KeyF32, false>), dim3(grid), dim3(WG), 0, st, GD_ARGS);
-    } else {
-        if (a.ghist2) hipLaunchKernelGGL((k_gather_default<KeyU32, true>), dim3(grid), dim3(WG), 0, st, GD_ARGS);
-        else hipLaunchKernelGGL((k_gather_default<KeyU32, false>), dim3(grid), dim3(WG), 0, st, GD_ARGS);
-    }
-#undef GD_ARGS
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------- range select + gather (beta path)
-constexpr int BETA_SPT = 4;                 // subranges per thread
-constexpr int BETA_TILE = WG * BETA_SPT;    // 1024 subranges per tile
-
-template <class K, int BETA, bool POS, bool HIST2>
-__global__ __launch_bounds__(WG) void k_range_gather_beta(const u32* __restrict__ keys, u64 n, int alpha, int filter,
-                                                          const u32* __restrict__ dvals, const void* __restrict__ doff,
-                                                          Ctrl* ctrl, u32* __restrict__ ckeys, u32* __restrict__ cidx,
-                                                          u64 cap, u64* status, u32* ticket, u32* ghist2, int hshift2,
-                                                          int bins2) {
-    __shared__ u64 scratch[WG / WAVE + 1];
-    __shared__ u64 tile_excl;
-    __shared__ u32 tslot;
-    __shared__ u32 lh[HIST2 ? 256 : 1];
-    __shared__ u64 red[3][WG / WAVE];
-    if (HIST2) {
-        for (int i = threadIdx.x; i < bins2; i += WG) lh[i] = 0;
-    }
-    const u32 thr = ctrl->first.result;
-    const u64 L = 1ull << alpha;
-    const u64 S = (n + L - 1) >> alpha;
-    const u64 ntiles = (S + BETA_TILE - 1) / BETA_TILE;
-    u64 nsel = 0, nlon = 0, nlel = 0;
-    for (;;) {
-        const u32 t = next_tile(ticket, &tslot);
-        if (t >= ntiles) break;
-        const u64 s0 = (u64)t * BETA_TILE + (u64)threadIdx.x * BETA_SPT;
-        // kind: 0 none, 1 selected (whole subrange, filtered), 2 lonely
-        int kind[BETA_SPT];
-        u32 cnt = 0;
-#pragma unroll
-        for (int j = 0; j < BETA_SPT; ++j) {
-            const u64 s = s0 + j;
-            kind[j] = 0;
-            if (s >= S) continue;
-            const u64 base = s << alpha;
-            const u64 len = min(L, n - base);
-            const int m = (int)std::min<u64>(BETA, len);
-            u32 d[BETA];
-#pragma unroll
-            for (int b = 0; b < BETA; ++b) d[b] = dvals[s * BETA + b];
-            if (d[0] < thr) continue;
-            bool all = true;
-#pragma unroll
-            for (int b = 0; b < BETA; ++b) all = all && (b >= m || d[b] >= thr);
-            if (all) {
-                kind[j] = 1;
-                ++nsel;
-                if (!filter) cnt += (u32)len;
-                else
-                    for (u64 i = base; i < base + len; ++i) cnt += K::to_ord(keys[i]) >= thr;
-            } else {
-                kind[j] = 2;
-                ++nlon;
-                nlel += (u64)m;
-                cnt += (u32)m;
-            }
-        }
-        u64 tot;
-        const u64 ex = block_excl_scan_u64<WG>((u64)cnt, scratch, &tot);
-        if (threadIdx.x == 0) tile_excl = lookback(status, t, tot, &ctrl->error);
-        __syncthreads();
-        u64 pos = tile_excl + ex;
-        auto put = [&](u32 o, u32 gidx) {
-            if (pos < cap) {
-                ckeys[pos] = o;
-                if (POS) cidx[pos] = gidx;
-            } else {
-                atomicOr(&ctrl->error, ERR_CAPACITY);
-            }
-            if (HIST2) atomicAdd(&lh[o >> hshift2], 1u);
-            ++pos;
-        };
-#pragma unroll
-        for (int j = 0; j < BETA_SPT; ++j) {
-            const u64 s = s0 + j;
-            if (kind[j] == 0) continue;
-            const u64 base = s << alpha;
-            const u64 len = min(L, n - base);
-            if (kind[j] == 1) {
-                for (u64 i = base; i < base + len; ++i) {
-                    const u32 o = K::to_ord(keys[i]);
-                    if (!filter || o >= thr) put(o, (u32)i);
-                }
-            } else {
-                const int m = (int)std::min<u64>(BETA, len);
-                u32 d[BETA], p[BETA];
-                u32 packed = 0;
-                if (POS) {
-                    if (BETA == 2) packed = ((const uint8_t*)doff)[s];
-                    else if (BETA == 3) packed = ((const uint16_t*)doff)[s];
-                    else packed = ((const u32*)doff)[s];
-                }
-#pragma unroll
-                for (int b = 0; b < BETA; ++b) {
-                    d[b] = dvals[s * BETA + b];
-                    const bool ge = b < m && d[b] >= thr;  // only these can reach the output
-                    p[b] = (POS && ge && b < BETA - 1) ? (u32)(base + ((packed >> (8 * b)) & 0xFFu)) : 0xFFFFFFFFu;
-                }
-                // delegates >= threshold in index order, then the rest (never selected later)
-#pragma unroll
-                for (int x = 0; x < BETA; ++x)
-#pragma unroll
-                    for (int y = 0; y + 1 < BETA - x; ++y)
-                        if (p[y + 1] < p[y]) {
-                            u32 tp = p[y]; p[y] = p[y + 1]; p[y + 1] = tp;
-                            u32 td = d[y]; d[y] = d[y + 1]; d[y + 1] = td;
-                        }
-                for (int b = 0; b < BETA; ++b)
-                    if (b < m) put(d[b], p[b]);
-            }
-        }
-        if (t == ntiles - 1 && threadIdx.x == 0) {
-            u64 C = tile_excl + tot;
-            ctrl->concat_size = C;
-            if (C > cap) {
-                atomicOr(&ctrl->error, ERR_CAPACITY);
-                C = cap;
-            }
-            ctrl->second.m = C;
-            if (C < ctrl->second.k) atomicOr(&ctrl->error, ERR_CONCAT_SHORT);
-        }
-        __syncthreads();
-    }
-    // counters
-    {
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        for (int o = 32; o > 0; o >>= 1) {
-            nsel += shfl_xor_u64(nsel, o);
-            nlon += shfl_xor_u64(nlon, o);
-            nlel += shfl_xor_u64(nlel, o);
-        }
-        if (lane == 0) { red[0][w] = nsel; red[1][w] = nlon; red[2][w] = nlel; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            u64 a = 0, b = 0, c = 0;
-            for (int i = 0; i < WG / WAVE; ++i) { a += red[0][i]; b += red[1][i]; c += red[2][i]; }
-            if (a) atomicAdd((unsigned long long*)&ctrl->selected, (unsigned long long)a);
-            if (b) atomicAdd((unsigned long long*)&ctrl->lonely_sub, (unsigned long long)b);
-            if (c) atomicAdd((unsigned long long*)&ctrl->lonely_el, (unsigned long long)c);
-        }
-    }
-    if (HIST2) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < bins2; i += WG) {
-            u32 c = lh[i];
-            if (c) atomicAdd(&ghist2[i], c);
-        }
-    }
-}
-
-template <class K, int BETA>
-static hipError_t launch_rgb_t(const BetaGatherArgs& a, hipStream_t st) {
-    const u64 L = 1ull << a.alpha;
-    const u64 S = (a.n + L - 1) >> a.alpha;
-    const u64 ntiles = (S + BETA_TILE - 1) / BETA_TILE;
-    unsigned grid = (unsigned)std::min<u64>(ntiles, 4096);
-#define RGB_ARGS a.keys, a.n, a.alpha, a.filter, a.dvals, a.doff, a.ctrl, a.ckeys, a.cidx, a.cap, a.status, a.ticket, \
-                 a.ghist2, a.hshift2, a.bins2
-    const bool pos = a.cidx != nullptr, h = a.ghist2 != nullptr;
-    if (pos && h) hipLaunchKernelGGL((k_range_gather_beta<K, BETA, true, true>), dim3(grid), dim3(WG), 0, st, RGB_ARGS);
-    else if (pos) hipLaunchKernelGGL((k_range_gather_beta<K, BETA, true, false>), dim3(grid), dim3(WG), 0, st, RGB_ARGS);
-    else if (h) hipLaunchKernelGGL((k_range_gather_beta<K, BETA, false, true>), dim3(grid), dim3(WG), 0, st, RGB_ARGS);
-    else hipLaunchKernelGGL((k_range_gather_beta<K, BETA, false, false>), dim3(grid), dim3(WG), 0, st, RGB_ARGS);
-#undef RGB_ARGS
-    return hipGetLastError();
-}
-template <class K>
-static hipError_t launch_rgb_k(const BetaGatherArgs& a, hipStream_t st) {
-    switch (a.beta) {
-        case 2: return launch_rgb_t<K, 2>(a, st);
-        case 3: return launch_rgb_t<K, 3>(a, st);
-        case 4: return launch_rgb_t<K, 4>(a, st);
-    }
-    return hipErrorInvalidValue;
-}
-hipError_t launch_range_gather_beta(const BetaGatherArgs& a, hipStream_t st) {
-    return a.f32 ? launch_rgb_k<KeyF32>(a, st) : launch_rgb_k<KeyU32>(a, st);
-}
-
-// ---------------------------------------------------------------- extraction
-template <class K, bool DIRECT>
-__global__ __launch_bounds__(WG) void k_extract(const u32* __restrict__ ckeys, const u32* __restrict__ cidx, u64 k,
-                                                SelState* sel, Ctrl* ctrl, u32* __restrict__ out_keys,
-                                                i64* __restrict__ out_idx, u64* status, u32* ticket) {
-    __shared__ u64 scratch[WG / WAVE + 1];
-    __shared__ u64 tile_excl;
-    __shared__ u32 tslot;
-    const u64 C = sel->m;
-    const u32 K0 = sel->result;
-    const u64 total_gt = sel->m - sel->below - sel->kcount;
-    const u64 r = k - total_gt;
-    const u64 ntiles = (C + TILE - 1) / TILE;
-    for (;;) {
-        const u32 t = next_tile(ticket, &tslot);
-        if (t >= ntiles) break;
-        const u64 e0 = (u64)t * TILE + (u64)threadIdx.x * TILE_E;
-        u32 v[TILE_E];
-        u32 gt = 0, eq = 0;
-#pragma unroll
-        for (int j = 0; j < TILE_E; ++j) {
-            const bool in = e0 + j < C;
-            v[j] = in ? (DIRECT ? K::to_ord(ckeys[e0 + j]) : ckeys[e0 + j]) : 0u;
-            gt += in && v[j] > K0;
-            eq += in && v[j] == K0;
-        }
-        u64 tot;
-        const u64 ex = block_excl_scan_u64<WG>(pack2(gt, eq), scratch, &tot);
-        if (threadIdx.x == 0) tile_excl = lookback(status, t, tot, &ctrl->error);
-        __syncthreads();
-        const u64 te = tile_excl;
-        u64 g = unpack_hi(te) + unpack_hi(ex), q = unpack_lo(te) + unpack_lo(ex);
-#pragma unroll
-        for (int j = 0; j < TILE_E; ++j) {
-            if (e0 + j >= C) break;
-            u64 pos = ~0ull;
-            if (v[j] > K0) {
-                pos = g + (q < r ? q : r);
-                ++g;
-            } else if (v[j] == K0) {
-                if (q < r) pos = g + q;
-                ++q;
-            }
-            if (pos != ~0ull) {
-                if (pos < k) {
-                    if (out_keys) out_keys[pos] = K::from_ord(v[j]);
-                    if (out_idx) out_idx[pos] = DIRECT ? (i64)(e0 + j) : (i64)cidx[e0 + j];
-                } else {
-                    atomicOr(&ctrl->error, ERR_CAPACITY);
-                }
-            }
-        }
-        __syncthreads();
-    }
-}
-
-hipError_t launch_extract(const ExtractArgs& a, hipStream_t st) {
-    const u64 ntiles = (a.cap + TILE - 1) / TILE;
-    unsigned grid = (unsigned)std::max<u64>(1, std::min<u64>(ntiles, 4096));
-#define EX_ARGS a.ckeys, a.cidx, a.k, a.sel, a.ctrl, (u32*)a.out_keys, a.out_idx, a.status, a.ticket
-    if (a.f32) {
-        if (a.direct) hipLaunchKernelGGL((k_extract<KeyF32, true>), dim3(grid), dim3(WG), 0, st, EX_ARGS);
-        else hipLaunchKernelGGL((k_extract<KeyF32, false>), dim3(grid), dim3(WG), 0, st, EX_ARGS);
-    } else {
-        if (a.direct) hipLaunchKernelGGL((k_extract<KeyU32, true>), dim3(grid), dim3(WG), 0, st, EX_ARGS);
-        else hipLaunchKernelGGL((k_extract<KeyU32, false>), dim3(grid), dim3(WG), 0, st, EX_ARGS);
-    }
-#undef EX_ARGS
+hipError_t launch_find_seg(const Seg& seg, SelState* sel, unsigned grid, hipStream_t st) {
+    hipLaunchKernelGGL((k_find<KeyU32, false, true>), dim3(grid), dim3(WG), 0, st, nullptr, seg, sel);
     return hipGetLastError();
 }
 
