@@ -42,6 +42,91 @@ def algorithmic_bytes(st: dict, n: int, k: int) -> int:
     return 4 * n + 4 * D + 4 * D * p1 + sel_read + 8 * C + 4 * C * p2 + 12 * k
 
 
+def batched_leg(dt, torch, steps: int, warmup: int, nrows: int = 4096, graph_loop: bool = True):
+    """Batched workload: nrows rows, lengths log-uniform 2^12..2^22, coin-flip uint32 uniform /
+    float32 N(0,1), k log-uniform in [1, min(n/8, 4096)], top-k keys + int64 indices per row.
+    One batched call (two launches) against a loop of single-vector calls over the same rows,
+    eagerly and captured in a HIP graph; outputs must be identical."""
+    from drtopksc_amd.workloads import batch_mix, pack
+    mix = batch_mix(seed=7, nrows=nrows)
+    rows, keys_len, out_len = pack(mix)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    buf = torch.randint(-2 ** 31, 2 ** 31 - 1, (keys_len,), dtype=torch.int32, device="cuda", generator=g)
+    for off, n, k, oo, d in rows:
+        if d == 1:
+            buf[off:off + n] = torch.randn(n, device="cuda", generator=g).view(torch.int32)
+    b = dt.Batch(rows, keys_len, out_len)
+    out = b.alloc_out()
+    stream = torch.cuda.current_stream()
+    for _ in range(max(1, warmup)):
+        b.run(buf, out=out, stream=stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        b.run(buf, out=out, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+
+    # loop of single-vector calls over the same rows (shared workspace, preallocated outputs)
+    views = []
+    for off, n, k, oo, d in rows:
+        x = buf[off:off + n]
+        views.append((x.view(torch.float32) if d == 1 else x, k, oo))
+    wsb = max(dt.workspace_size(n, k, dt.ALGO_RADIX, d) for _, n, k, _, d in rows)
+    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    lk = torch.empty(len(rows), dtype=torch.int32, device="cuda")
+    lkeys = torch.empty(max(out_len, 1), dtype=torch.int32, device="cuda")
+    lidx = torch.empty(max(out_len, 1), dtype=torch.int64, device="cuda")
+
+    def loop(st):
+        for r, (x, k, oo) in enumerate(views):
+            o = (lk[r:r + 1].view(x.dtype), lkeys[oo:oo + k].view(x.dtype), lidx[oo:oo + k])
+            dt.select(x, k, workspace=ws, out=o, stream=st)
+
+    loop(stream)
+    torch.cuda.synchronize()
+    identical = bool(torch.equal(lk, out[0]) and torch.equal(lkeys[:out_len], out[1][:out_len])
+                     and torch.equal(lidx[:out_len], out[2][:out_len]))
+    t0 = time.perf_counter()
+    e0.record(stream)
+    loop(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    loop_ms = e0.elapsed_time(e1)
+    loop_wall_ms = (time.perf_counter() - t0) * 1e3
+    loop_graph_ms = None
+    if graph_loop:
+        s = torch.cuda.Stream()
+        s.wait_stream(stream)
+        gr = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gr, stream=s):
+            loop(torch.cuda.current_stream())
+        gr.replay()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        gr.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        loop_graph_ms = e0.elapsed_time(e1)
+        del gr
+    total = sum(n for _, n, _, _, _ in rows)
+    best_loop = min(x for x in (loop_ms, loop_graph_ms) if x is not None)
+    res = {"workload": f"{len(rows)} rows, n log-uniform 2^12..2^22, uint32 uniform / float32 N(0,1) coin flip, "
+                       "k log-uniform [1, min(n/8, 4096)], keys + int64 indices out",
+           "rows": len(rows), "keys": total, "ms": round(ms, 4), "req_s": round(len(rows) / (ms * 1e-3), 1),
+           "gelem_s": round(total / (ms * 1e-3) / 1e9, 1),
+           "hbm_frac_4n": round(4 * total / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+           "loop_ms": round(loop_ms, 3), "loop_wall_ms": round(loop_wall_ms, 3),
+           "loop_graph_ms": round(loop_graph_ms, 3) if loop_graph_ms is not None else None,
+           "speedup_vs_loop": round(best_loop / ms, 2), "outputs_identical": identical}
+    del buf, ws
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,6 +135,8 @@ def main():
     ap.add_argument("--log2n", type=int, default=30)
     ap.add_argument("--kmax-log2", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-rows", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU exchange is "
@@ -154,6 +241,13 @@ def main():
                "sample": f"full 2^{args.log2n} uniform vector, k=2^10 (nth_element per chunk + merge + sort + "
                          f"index pass), {t_cpu:.2f}s"}
 
+    batched = None
+    if not args.no_batched:
+        del dev, ws
+        torch.cuda.empty_cache()
+        batched = batched_leg(dt, torch, args.steps, args.warmup, nrows=args.batch_rows)
+        print(f"[bench] batched {batched}", file=sys.stderr, flush=True)
+
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Gelem/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
@@ -162,7 +256,7 @@ def main():
                                f"k=2^0..2^{args.kmax_log2}, keys+int64 indices out",
                    "n": n, "k_sweep": [ks[0], ks[-1]], "algo": "radix", "flags": "FILTER|BETA|SHUFFLE",
                    "digit_bits": 8, "beta": 2, "alpha": "auto (reference policy)"},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
         "min_hbm_frac": min(r["hbm_frac"] for r in rows),
         "per_k": rows, "wall_s": round(wall, 3),
     }

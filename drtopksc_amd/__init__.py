@@ -7,5 +7,6 @@ if the library is missing or no GPU is present.
 """
 from .api import (  # noqa: F401
     ALGO_BITONIC, ALGO_BUCKET, ALGO_RADIX, FLAG_BETA, FLAG_FILTER, FLAG_SHUFFLE, FLAG_TIMING, FLAGS_DEFAULT,
-    DrTopKError, Params, Stats, lib, lib_path, params, sample_delegates, select, workspace_size,
+    TUNE_DEFAULT, U32, F32, Batch, DrTopKError, Params, Row, Stats, Tuning, lib, lib_path, params, sample_delegates,
+    select, tuning_default, tuning_from_table, workspace_size,
 )

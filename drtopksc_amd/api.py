@@ -8,6 +8,9 @@ HIP kernels of the library.  Mirrors the reference's entry points:
                                  (bucket_select/bucketselect_combined.cuh:1280, :473)
   select(..., algo=ALGO_BITONIC) bitonic_firstTopk + bitonic (bitonic/bitonic_com.cuh:610, :669)
   sample_delegates(...)          sampleMax / sampleMax_NoReduction_Beta* (radixselect.cuh:883, :967)
+  Batch(...)                     many rows of uneven length in one call (no reference counterpart;
+                                 the reference's MultiGPU driver loops single vectors,
+                                 MultiGPU/main.cu:366-424)
 """
 from __future__ import annotations
 
@@ -54,6 +57,19 @@ class Stats(ctypes.Structure):
         return d
 
 
+class Row(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint64), ("n", ctypes.c_uint64), ("k", ctypes.c_uint64),
+                ("out_offset", ctypes.c_uint64), ("dtype", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("alpha", (ctypes.c_uint8 * 32) * 32), ("digit_bits", (ctypes.c_uint8 * 32) * 32),
+                ("tile_log2", (ctypes.c_uint8 * 32) * 32)]
+
+
+TUNE_DEFAULT = 0xFF
+
+
 def params(alpha=-1, beta=2, digit_bits=8, num_buckets=256, flags=FLAGS_DEFAULT) -> Params:
     return Params(alpha, beta, digit_bits, num_buckets, flags, (ctypes.c_uint32 * 3)(0, 0, 0))
 
@@ -83,6 +99,13 @@ def lib():
         L.drtopk_select.restype = ctypes.c_int
         L.drtopk_sample_delegates.argtypes = [ctypes.c_int, vp, u64, i32, i32, u32, vp, vp, vp]
         L.drtopk_sample_delegates.restype = ctypes.c_int
+        L.drtopk_batch_plan.argtypes = [ctypes.POINTER(Row), u64, u64, u64, ctypes.POINTER(Tuning), vp,
+                                        ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+        L.drtopk_batch_plan.restype = ctypes.c_int
+        L.drtopk_select_batched.argtypes = [vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, vp]
+        L.drtopk_select_batched.restype = ctypes.c_int
+        L.drtopk_tuning_default.argtypes = [ctypes.POINTER(Tuning)]
+        L.drtopk_tuning_default.restype = None
         L.drtopk_last_error.argtypes = []
         L.drtopk_last_error.restype = ctypes.c_char_p
         L.drtopk_version.argtypes = []
@@ -168,3 +191,66 @@ def sample_delegates(keys, alpha: int, beta: int = 1, shuffle: bool = False, wan
                                        _stream_ptr(stream))
     _check(rc)
     return vals, pos
+
+
+def tuning_default() -> Tuning:
+    t = Tuning()
+    lib().drtopk_tuning_default(ctypes.byref(t))
+    return t
+
+
+def tuning_from_table(table: dict) -> Tuning:
+    """Tuning from the autotuner's JSON table: {"cells": [[log2n, log2k, alpha, digit_bits, tile_log2], ...]}."""
+    t = tuning_default()
+    for ln, lk, a, nb, tl in table.get("cells", []):
+        t.alpha[ln][lk] = a
+        t.digit_bits[ln][lk] = nb
+        t.tile_log2[ln][lk] = tl
+    return t
+
+
+class Batch:
+    """A planned batch of rows (drtopk_batch_plan), resident on the device.
+
+    rows: sequence of (offset, n, k, out_offset, dtype) with dtype U32 or F32; offsets in
+    32-bit elements of the shared key buffer (multiples of 4).  run() launches
+    drtopk_select_batched on a stream and returns (kth [nrows] int32 bits, topk keys
+    [out_len] int32 bits, topk indices [out_len] int64), row r's outputs at its out_offset.
+    """
+
+    def __init__(self, rows, keys_len: int, out_len: int, tuning: Tuning | None = None, device="cuda"):
+        import numpy as np
+        import torch
+        self.nrows = len(rows)
+        arr = (Row * self.nrows)()
+        for i, (off, n, k, oo, dt) in enumerate(rows):
+            arr[i] = Row(int(off), int(n), int(k), int(oo), int(dt), 0)
+        self._rows = arr
+        self.keys_len, self.out_len = int(keys_len), int(out_len)
+        tp = ctypes.byref(tuning) if tuning is not None else None
+        pb, wb = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(lib().drtopk_batch_plan(arr, self.nrows, self.keys_len, self.out_len, tp, None, ctypes.byref(pb),
+                                       ctypes.byref(wb)))
+        self.h_plan = np.zeros(int(pb.value), dtype=np.uint8)
+        _check(lib().drtopk_batch_plan(arr, self.nrows, self.keys_len, self.out_len, tp,
+                                       ctypes.c_void_p(self.h_plan.ctypes.data), ctypes.byref(pb), ctypes.byref(wb)))
+        self.ws_bytes = int(wb.value)
+        self.d_plan = torch.from_numpy(self.h_plan).to(device)
+        self.ws = torch.empty(max(self.ws_bytes, 256), dtype=torch.uint8, device=device)
+
+    def alloc_out(self, device="cuda"):
+        import torch
+        return (torch.empty(self.nrows, dtype=torch.int32, device=device),
+                torch.empty(max(self.out_len, 1), dtype=torch.int32, device=device),
+                torch.empty(max(self.out_len, 1), dtype=torch.int64, device=device))
+
+    def run(self, keys, out=None, stream=None):
+        if keys.numel() < self.keys_len or keys.element_size() != 4 or not keys.is_cuda:
+            raise DrTopKError(1, "keys must be a device tensor of >= keys_len 32-bit elements")
+        kth, tk, ti = out if out is not None else self.alloc_out(keys.device)
+        _check(lib().drtopk_select_batched(ctypes.c_void_p(self.h_plan.ctypes.data),
+                                           ctypes.c_void_p(self.d_plan.data_ptr()), ctypes.c_void_p(keys.data_ptr()),
+                                           ctypes.c_void_p(self.ws.data_ptr()), self.ws.numel(),
+                                           ctypes.c_void_p(kth.data_ptr()), ctypes.c_void_p(tk.data_ptr()),
+                                           ctypes.c_void_p(ti.data_ptr()), _stream_ptr(stream)))
+        return kth, tk, ti

@@ -17,15 +17,16 @@
 
 #include "../../include/drtopk.h"
 #include "common.hpp"
+#include "host_util.hpp"
 #include "kernels.hpp"
 
 using namespace drtopk;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-drtopk_status fail(drtopk_status s, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+namespace drtopk {
 drtopk_status fail(drtopk_status s, const char* fmt, ...) {
     char buf[512];
     va_list ap;
@@ -35,11 +36,10 @@ drtopk_status fail(drtopk_status s, const char* fmt, ...) {
     g_last_error = buf;
     return s;
 }
+void clear_error() { g_last_error.clear(); }
+}  // namespace drtopk
 
-int floor_log2(u64 x) { int l = -1; while (x) { x >>= 1; ++l; } return l; }
-int ceil_log2(u64 x) { int l = floor_log2(x); return ((1ull << l) == x) ? l : l + 1; }
-u64 align_up(u64 x, u64 a) { return (x + a - 1) / a * a; }
-u64 cdiv(u64 a, u64 b) { return (a + b - 1) / b; }
+namespace {
 
 constexpr u64 MAX_N = (1ull << 31) - 1;
 
@@ -64,6 +64,7 @@ struct Plan {
     u64 o_ctrl, o_hist, o_zero_end;
     u64 o_rcnt, o_roff, o_ecnt, o_eoff, o_scnt;
     u64 o_dvals, o_doff, o_sel, o_skeys, o_sidx, total;
+    u64 npad1, npad2, o_bufA, o_bufB, o_dsel;  // bitonic family
 };
 
 // Parameter policy: radix alpha = floor(0.5*(log2|V| - log2 k + 3)), beta path iff alpha<=5
@@ -82,7 +83,8 @@ drtopk_status resolve(drtopk_algo algo, drtopk_dtype dtype, u64 n, u64 k, const 
     if (p->num_buckets < 4 || p->num_buckets > 4096) return fail(DRTOPK_ERR_INVALID_ARG, "num_buckets %d not in [4, 4096]", p->num_buckets);
     if (p->beta < 1 || p->beta > 4) return fail(DRTOPK_ERR_INVALID_ARG, "beta %d not in [1, 4]", p->beta);
     if (p->alpha < -1 || p->alpha > 30) return fail(DRTOPK_ERR_INVALID_ARG, "alpha %d not in [-1, 30]", p->alpha);
-    if (algo == DRTOPK_ALGO_BITONIC) return fail(DRTOPK_ERR_UNSUPPORTED, "bitonic family not built yet");
+    if (algo == DRTOPK_ALGO_BITONIC && (k & (k - 1)))
+        return fail(DRTOPK_ERR_INVALID_ARG, "bitonic top-k needs a power-of-two k (got %llu)", (unsigned long long)k);
     const int Ln = floor_log2(n);
     int alpha = p->alpha;
     const bool autoa = alpha < 0;
@@ -123,15 +125,21 @@ drtopk_status resolve(drtopk_algo algo, drtopk_dtype dtype, u64 n, u64 k, const 
     // layout
     pl->segcap = 0;
     pl->nseg = 0;
-    if (path == 1) {
+    pl->npad1 = pl->npad2 = 0;
+    const bool bit = algo == DRTOPK_ALGO_BITONIC;
+    auto pow2 = [](u64 x) { u64 p = 1; while (p < x) p <<= 1; return p; };
+    if (bit) {
+        pl->npad1 = path ? pow2(pl->S) : 0;
+        pl->npad2 = pow2(path ? (k << alpha) : n);
+    } else if (path == 1) {
         pl->segcap = gather_segcap(k * pl->L);
         pl->nseg = gather_nseg(k * pl->L);
     } else if (path == 2) {
         pl->segcap = beta_segcap(pl->S, alpha);
         pl->nseg = beta_nseg(pl->S);
     }
-    pl->ext_nseg = path == 0 ? cdiv(n, DIRECT_TILE) : pl->nseg;
-    pl->range_nt = path == 1 ? range_tiles(pl->S) : 0;
+    pl->ext_nseg = bit ? 0 : (path == 0 ? cdiv(n, DIRECT_TILE) : pl->nseg);
+    pl->range_nt = (path == 1 && !bit) ? range_tiles(pl->S) : 0;
     pl->hist_bins = std::max<int>(HIST_REPS * 256, std::max<int>(1 << nb, pl->B));
     u64 o = 0;
     pl->o_ctrl = o; o = align_up(o + sizeof(Ctrl), 256);
@@ -145,9 +153,13 @@ drtopk_status resolve(drtopk_algo algo, drtopk_dtype dtype, u64 n, u64 k, const 
     pl->o_dvals = o; o = align_up(o + 4ull * pl->D, 256);
     const u64 offb = path == 2 ? (beta == 2 ? 1 : beta == 3 ? 2 : 4) : 0;
     pl->o_doff = o; o = align_up(o + offb * pl->S, 256);
-    pl->o_sel = o; o = align_up(o + (path == 1 ? 4ull * k : 0), 256);
+    pl->o_sel = o; o = align_up(o + ((path == 1 && !bit) ? 4ull * k : 0), 256);
     pl->o_skeys = o; o = align_up(o + 4ull * pl->segcap * pl->nseg, 256);
     pl->o_sidx = o; o = align_up(o + 4ull * pl->segcap * pl->nseg, 256);
+    const u64 nbuf = std::max(pl->npad1, pl->npad2);
+    pl->o_bufA = o; o = align_up(o + 8ull * nbuf, 256);
+    pl->o_bufB = o; o = align_up(o + 8ull * nbuf, 256);
+    pl->o_dsel = o; o = align_up(o + (bit && path ? 8ull * k : 0), 256);
     pl->total = o;
     return DRTOPK_OK;
 }
@@ -175,12 +187,6 @@ SelState make_sel(int mode, u64 m, u64 k, int nb, int B, int stop_digit, int sto
 int bucket_max_levels(int B) {
     return (int)std::ceil(32.0 / std::log2((double)(B - 1))) + 2;
 }
-
-#define HIPCHK(x)                                                                              \
-    do {                                                                                       \
-        hipError_t e_ = (x);                                                                   \
-        if (e_ != hipSuccess) return fail(DRTOPK_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
-    } while (0)
 
 }  // namespace
 
@@ -213,7 +219,7 @@ drtopk_status drtopk_workspace_size(drtopk_algo algo, drtopk_dtype dtype, uint64
 drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_keys, uint64_t n, uint64_t k,
                             const drtopk_params* p, void* d_workspace, size_t workspace_bytes, void* d_kth,
                             void* d_topk_keys, int64_t* d_topk_idx, drtopk_stats* h_stats, void* stream) {
-    g_last_error.clear();
+    clear_error();
     Plan pl;
     drtopk_status s = resolve(algo, dtype, n, k, p, &pl);
     if (s != DRTOPK_OK) return s;
@@ -269,7 +275,22 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
     HIPCHK(launch_init(ctrl, init, (u32*)(ws + pl.o_hist), (pl.o_zero_end - pl.o_hist) / 4, st));
     HIPCHK(mark(0));
 
-    if (pl.path == 0) {
+    if (pl.algo == DRTOPK_ALGO_BITONIC) {
+        // ---- bitonic family (bitonic/sampleBitonic.cu:283-308)
+        if (pl.path != 0) {
+            SampleArgs sa;
+            sa.keys = keys; sa.n = pl.n; sa.alpha = pl.alpha; sa.beta = 1; sa.f32 = pl.f32;
+            sa.staged = 0; sa.posmode = 0; sa.fuse = 0;
+            sa.dvals = dvals; sa.dpos = nullptr; sa.ghist = nullptr; sa.hshift = 0; sa.bins = 0; sa.ctrl = ctrl;
+            HIPCHK(launch_sample(sa, st));
+        }
+        HIPCHK(mark(1));
+        BitonicArgs ba{keys, pl.n, pl.alpha, pl.f32, pl.k, pl.S, dvals, ctrl, (u64*)(ws + pl.o_bufA),
+                       (u64*)(ws + pl.o_bufB), (u64*)(ws + pl.o_dsel), pl.npad1, pl.npad2, d_topk_keys, d_topk_idx,
+                       d_kth};
+        HIPCHK(launch_bitonic(ba, st));
+        for (int i = 2; i <= 4; ++i) HIPCHK(mark(i));
+    } else if (pl.path == 0) {
         // ---- no delegates: radix_select / bucket_select over V (radixselect.cuh:1473,
         //      bucketselect_combined.cuh:473)
         HIPCHK(mark(1));
@@ -411,7 +432,7 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
 
 drtopk_status drtopk_sample_delegates(drtopk_dtype dtype, const void* d_keys, uint64_t n, int32_t alpha,
                                       int32_t beta, uint32_t flags, void* d_vals, uint32_t* d_pos, void* stream) {
-    g_last_error.clear();
+    clear_error();
     if (dtype != DRTOPK_U32 && dtype != DRTOPK_F32) return fail(DRTOPK_ERR_INVALID_ARG, "bad dtype");
     if (!d_keys || !d_vals) return fail(DRTOPK_ERR_INVALID_ARG, "NULL pointer");
     if (((uintptr_t)d_keys & 15) != 0) return fail(DRTOPK_ERR_INVALID_ARG, "d_keys must be 16-byte aligned");

@@ -1,0 +1,650 @@
+// batch.hip -- batched top-k over many rows of uneven length, each with its own k and key
+// type (drtopk_batch_plan / drtopk_select_batched).
+//
+// The reference handles one vector per call and drives every phase from the host
+// (radixselect.cuh:1783-1894); its MultiGPU driver loops that per vector
+// (MultiGPU/main.cu:366-424).  Here a whole batch is two launches:
+//
+//  k_bsample  every row's delegate pass (one maximum per subrange of 2^alpha keys,
+//             sampleMax, radixselect.cuh:883-918) as one flat, persistent grid over
+//             fixed-size tiles of all rows, 16-byte loads, eight in flight per lane.
+//             It is the only kernel that reads all keys.
+//  k_brows    one workgroup per row (persistent, rows taken largest-first from a queue):
+//             first top-k over the row's delegates (exact k-th largest delegate by radix
+//             digits), range selection (every subrange whose delegate is greater, then the
+//             lowest-ID ties up to k, radixselect.cuh:413-427 / :603-618), filtered
+//             concatenation of those subranges (keys >= threshold, radixselect.cuh:1575-1607)
+//             into LDS, second top-k over it and the index-ordered extraction of the
+//             top-k keys and indices (MultiGPU/radixselect.cuh:1355-1383).  When the
+//             concatenation does not fit in LDS it is re-read from the row's keys through the
+//             selected-subrange list on every pass instead (bounded memory, same result).
+//
+// Why the threshold works (the same argument as the single-vector path): the k-th largest
+// delegate T is <= the row's k-th largest key K; every key > T sits in a subrange whose
+// delegate is > T, and when K == T the lowest-ID subranges with delegate == T hold the
+// lowest-index keys equal to K that the top-k set needs.  So the filtered concatenation
+// holds every key of the top-k set and K is its k-th largest.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../../include/drtopk.h"
+#include "common.hpp"
+#include "host_util.hpp"
+
+namespace drtopk {
+namespace {
+
+constexpr u64 PLAN_MAGIC = 0x3142544b50545244ull;  // "DRTPKTB1"
+
+// Plan blob: header, rows, chunk -> row map, row processing order.  Offsets are bytes
+// from the start of the blob, so the same bytes work on the host and on the device.
+struct BatchHeader {
+    u64 magic;
+    u64 nrows;
+    u64 nchunks;
+    u64 o_rows, o_chunk_row, o_order;
+    u64 plan_bytes;
+    u64 ws_bytes;
+    u64 o_ws_deleg, o_ws_sel;  // workspace byte offsets: delegates, selected subranges
+    u64 keys_len, out_len;
+    u32 sample_grid, rows_grid;
+    u32 pad[6];
+};
+static_assert(sizeof(BatchHeader) == 128, "header layout");
+
+struct BatchRow {
+    u64 key_off;    // first key in d_keys (multiple of 4)
+    u64 n, k;
+    u64 out_off;    // first output slot
+    u64 deleg_off;  // delegates in the workspace delegate array (multiple of 4)
+    u64 sel_off;    // selected subrange ids in the workspace id array
+    u32 S;          // subranges (0 when alpha == 0)
+    u32 chunk_begin, nchunks;
+    u8 alpha, f32, nb, tile_log2;
+};
+static_assert(sizeof(BatchRow) == 64, "row layout");
+
+// ---------------------------------------------------------------- delegate pass
+constexpr int BS_NT = 256;
+constexpr int BS_UNR = 8;  // 16-byte loads in flight per lane
+constexpr int MAX_TILE_LOG2 = 15;
+
+template <bool F32>
+__device__ __forceinline__ u32 ord(u32 b) {
+    return F32 ? KeyF32::to_ord(b) : b;
+}
+
+// One tile (<= 2^15 keys) of one row: subrange maxima into dout.
+template <bool F32>
+__device__ __forceinline__ void bsample_tile(const u32* __restrict__ src, u64 len, int a, u32* __restrict__ dout,
+                                             u32* part) {
+    const int t = threadIdx.x, lane = t & (WAVE - 1);
+    const u32 nvec = (u32)((len + 3) >> 2);
+    const u32 iters = (nvec + BS_NT - 1) / BS_NT;
+    for (u32 i0 = 0; i0 < iters; i0 += BS_UNR) {
+        u32x4 q[BS_UNR];
+        if ((u64)(i0 + BS_UNR) * BS_NT * 4 <= len) {
+#pragma unroll
+            for (int u = 0; u < BS_UNR; ++u)
+                q[u] = __builtin_nontemporal_load((const u32x4*)(src + ((u64)(i0 + u) * BS_NT + t) * 4));
+#pragma unroll
+            for (int u = 0; u < BS_UNR; ++u)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) q[u][j] = ord<F32>(q[u][j]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < BS_UNR; ++u) {
+                const u64 e = ((u64)(i0 + u) * BS_NT + t) * 4;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) q[u][j] = (e + j < len) ? ord<F32>(src[e + j]) : 0u;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < BS_UNR; ++u) {
+            if (i0 + u >= iters) break;  // uniform over the workgroup
+            const u64 e = ((u64)(i0 + u) * BS_NT + t) * 4;
+            if (a == 1) {
+                if (e < len) dout[e >> 1] = max(q[u][0], q[u][1]);
+                if (e + 2 < len) dout[(e >> 1) + 1] = max(q[u][2], q[u][3]);
+            } else if (a <= 8) {
+                u32 m = max(max(q[u][0], q[u][1]), max(q[u][2], q[u][3]));
+                const int g = 1 << (a - 2);  // lanes per subrange
+                for (int o = 1; o < g; o <<= 1) m = max(m, (u32)__shfl_xor((int)m, o, WAVE));
+                if ((lane & (g - 1)) == 0 && e < len) dout[e >> a] = m;
+            } else {
+                const u32 m = wave_max_u32(max(max(q[u][0], q[u][1]), max(q[u][2], q[u][3])));
+                if (lane == 0) part[((i0 + u) * BS_NT + t) >> 6] = m;  // 256-key block maximum
+            }
+        }
+    }
+    if (a > 8) {
+        __syncthreads();
+        const u32 per = 1u << (a - 8);
+        const u32 nblk = (u32)((len + 255) >> 8);
+        const u32 nsub = (u32)((len + (1ull << a) - 1) >> a);
+        for (u32 s = t; s < nsub; s += BS_NT) {
+            u32 m = 0;
+            for (u32 j = 0; j < per; ++j) {
+                const u32 b = s * per + j;
+                if (b < nblk) m = max(m, part[b]);
+            }
+            dout[s] = m;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(BS_NT) void k_bsample(const unsigned char* __restrict__ plan, const u32* __restrict__ keys,
+                                                   u32* __restrict__ wsq, u32* __restrict__ deleg) {
+    const BatchHeader* h = (const BatchHeader*)plan;
+    const BatchRow* rows = (const BatchRow*)(plan + h->o_rows);
+    const u32* crow = (const u32*)(plan + h->o_chunk_row);
+    const u64 nch = h->nchunks;
+    if (blockIdx.x == 0 && threadIdx.x == 0) wsq[0] = 0;  // row queue of k_brows (next launch)
+    __shared__ u32 part[1 << (MAX_TILE_LOG2 - 8)];
+    for (u64 c = blockIdx.x; c < nch; c += gridDim.x) {
+        const BatchRow* rp = rows + crow[c];
+        const int a = rp->alpha, tl = rp->tile_log2;
+        const u64 n = rp->n;
+        const u64 start = (c - rp->chunk_begin) << tl;
+        const u64 len = min((u64)1 << tl, n - start);
+        const u32* src = keys + rp->key_off + start;
+        u32* dout = deleg + rp->deleg_off + (start >> a);
+        if (rp->f32) bsample_tile<true>(src, len, a, dout, part);
+        else bsample_tile<false>(src, len, a, dout, part);
+    }
+}
+
+// ---------------------------------------------------------------- per-row workgroup
+constexpr int BR_NT = 512;
+constexpr int BR_W = BR_NT / WAVE;
+constexpr u32 REG_WORDS = 12288;       // 48 KB: candidate list or concatenation cache
+constexpr u32 CONCAT_CAP = REG_WORDS / 2;  // keys + row indices
+
+// Sources: a sequence of E slots read four at a time (e a multiple of 4); each slot yields
+// an ordered key, its row index (or element index) and a valid bit.
+struct SrcG {  // global ordered u32 array (delegates), 16-byte aligned
+    const u32* p;
+    u64 m;
+    __device__ __forceinline__ u32 load4(u64 e, u32* x, u32* ix) const {
+        u32 v = 0;
+        if (e + 4 <= m) {
+            const u32x4 q = *(const u32x4*)(p + e);
+            x[0] = q[0]; x[1] = q[1]; x[2] = q[2]; x[3] = q[3];
+            v = 0xF;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool ok = e + j < m;
+                x[j] = ok ? p[e + j] : 0u;
+                v |= (ok ? 1u : 0u) << j;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ix[j] = (u32)(e + j);
+        return v;
+    }
+};
+
+struct SrcL {  // LDS keys + row indices, 16-byte aligned
+    const u32* key;
+    const u32* idx;
+    u32 m;
+    __device__ __forceinline__ u32 load4(u64 e, u32* x, u32* ix) const {
+        u32 v = 0;
+        if (e + 4 <= m) {
+            const u32x4 q = *(const u32x4*)(key + e);
+            const u32x4 r = *(const u32x4*)(idx + e);
+            x[0] = q[0]; x[1] = q[1]; x[2] = q[2]; x[3] = q[3];
+            ix[0] = r[0]; ix[1] = r[1]; ix[2] = r[2]; ix[3] = r[3];
+            v = 0xF;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool ok = e + j < m;
+                x[j] = ok ? key[e + j] : 0u;
+                ix[j] = ok ? idx[e + j] : 0u;
+                v |= (ok ? 1u : 0u) << j;
+            }
+        }
+        return v;
+    }
+};
+
+// The row's keys >= lo, read through the selected-subrange list (alpha > 0: slot e is key
+// (sel[e >> alpha] << alpha) + (e & (2^alpha - 1))) or directly (alpha == 0: slot e is key e).
+struct SrcV {
+    const u32* row;
+    u64 n;
+    const u32* sel;
+    int alpha;
+    int f32;
+    u32 lo;
+    u64 E;  // slots (alpha == 1 reads the id list slot by slot and must stop at E)
+    __device__ __forceinline__ u32 cvt(u32 b) const { return f32 ? KeyF32::to_ord(b) : b; }
+    __device__ __forceinline__ u32 load4(u64 e, u32* x, u32* ix) const {
+        u32 v = 0;
+        if (alpha != 1) {
+            const u64 base = alpha == 0 ? e : (((u64)sel[e >> alpha] << alpha) | (e & ((1ull << alpha) - 1)));
+            if (base + 4 <= n) {
+                const u32x4 q = *(const u32x4*)(row + base);
+                x[0] = q[0]; x[1] = q[1]; x[2] = q[2]; x[3] = q[3];
+                v = 0xF;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const bool ok = base + j < n;
+                    x[j] = ok ? row[base + j] : 0u;
+                    v |= (ok ? 1u : 0u) << j;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) ix[j] = (u32)(base + j);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const u64 ee = e + j;
+                const u64 base = ee < E ? (((u64)sel[ee >> 1] << 1) | (ee & 1)) : n;
+                const bool ok = base < n;
+                x[j] = ok ? row[base] : 0u;
+                ix[j] = (u32)base;
+                v |= (ok ? 1u : 0u) << j;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            x[j] = cvt(x[j]);
+            if (x[j] < lo) v &= ~(1u << j);
+        }
+        return v;
+    }
+};
+
+struct RowLds {
+    u32 hist[BR_W * 256];
+    u64 scan[BR_W + 1];
+    u32 misc[16];
+};
+
+// k-th largest (ascending rank m-k) of the valid slots of src by radix digits of nb bits,
+// most significant first.  With `gather`, once the candidates of the current prefix fit in
+// cand[0..cap) the next pass copies them there and the remaining digits run on LDS.
+// Returns the key; *gt = valid slots strictly greater.
+template <class Src>
+__device__ u32 wg_kth(const Src& src, u64 E, u64 k, int nb, bool gather_ok, u32* cand, u32 cap, RowLds& L,
+                      u64* gt_out) {
+    const int t = threadIdx.x, w = t / WAVE, lane = t & (WAVE - 1);
+    const int R = 1 << nb;
+    u32 prefix = 0, mask = 0;
+    u64 kk = k, gt = 0;
+    bool in_cand = false, gather = gather_ok && E <= cap;
+    u32 mc = 0;
+    if (t == 0) L.misc[0] = 0;  // gather counter
+    for (int shift = 32 - nb; shift >= 0; shift -= nb) {
+        for (int i = t; i < BR_W * R; i += BR_NT) L.hist[(i / R) * 256 + (i % R)] = 0;
+        __syncthreads();
+        u32* wh = L.hist + w * 256;
+        if (in_cand) {
+            for (u32 i0 = 0; i0 < mc; i0 += BR_NT) {
+                const u32 i = i0 + t;
+                const u32 x = i < mc ? cand[i] : 0u;
+                wave_hist_add(wh, i < mc && (x & mask) == prefix, (x >> shift) & (R - 1));
+            }
+        } else {
+            for (u64 e0 = 0; e0 < E; e0 += (u64)BR_NT * 4) {
+                const u64 e = e0 + (u64)t * 4;
+                u32 x[4], ix[4];
+                const u32 v = e < E ? src.load4(e, x, ix) : 0u;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const bool act = ((v >> j) & 1u) && (x[j] & mask) == prefix;
+                    wave_hist_add(wh, act, (x[j] >> shift) & (R - 1));
+                    if (gather && act) {
+                        const u32 p = atomicAdd(&L.misc[0], 1u);
+                        if (p < cap) cand[p] = x[j];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (gather) {
+            in_cand = true;
+            gather = false;
+            mc = min(L.misc[0], cap);
+        }
+        if (t < R) {
+            u32 s = 0;
+            for (int q = 0; q < BR_W; ++q) s += L.hist[q * 256 + t];
+            L.hist[t] = s;
+        }
+        __syncthreads();
+        if (w == 0) {
+            // bins from the top: lane l holds bins R-1-l*bpl ... R-(l+1)*bpl
+            const int bpl = R >= WAVE ? R / WAVE : 1;
+            u32 c[4] = {0, 0, 0, 0};
+            u64 s = 0;
+            for (int j = 0; j < bpl; ++j) {
+                const int b = R - 1 - lane * bpl - j;
+                c[j] = b >= 0 ? L.hist[b] : 0u;
+                s += c[j];
+            }
+            const u64 inc = wave_incl_scan_u64(s);
+            const u64 bal = __ballot(inc >= kk);
+            const int fl = bal ? __ffsll((unsigned long long)bal) - 1 : 0;
+            if (lane == fl) {
+                u64 above = inc - s;
+                int bin = R - 1 - lane * bpl - (bpl - 1);
+                u32 cnt = c[bpl - 1];
+                for (int j = 0; j < bpl; ++j) {
+                    if (above + c[j] >= kk) {
+                        bin = R - 1 - lane * bpl - j;
+                        cnt = c[j];
+                        break;
+                    }
+                    above += c[j];
+                }
+                L.misc[2] = (u32)bin;
+                L.misc[3] = cnt;
+                L.misc[4] = (u32)above;
+                L.misc[5] = (u32)(above >> 32);
+            }
+        }
+        __syncthreads();
+        const u32 bin = L.misc[2], cnt = L.misc[3];
+        const u64 above = (u64)L.misc[4] | ((u64)L.misc[5] << 32);
+        prefix |= bin << shift;
+        mask |= (u32)(R - 1) << shift;
+        kk -= above;
+        gt += above;
+        if (!in_cand && gather_ok && cnt <= cap && shift > 0) gather = true;
+    }
+    *gt_out = gt;
+    return prefix;
+}
+
+// Ordered pass: calls emit(pos, key, index) for every valid slot with key > K and for the
+// first `ties` valid slots with key == K (ALL: every valid slot), pos = rank among those
+// emitted, in slot order.  Returns the number of valid slots > K (ALL: valid slots).
+template <bool ALL, class Src, class F>
+__device__ u64 wg_ordered(const Src& src, u64 E, u32 K, u64 ties, RowLds& L, F emit) {
+    const int t = threadIdx.x;
+    u64 gt_run = 0, eq_run = 0;
+    for (u64 e0 = 0; e0 < E; e0 += (u64)BR_NT * 4) {
+        const u64 e = e0 + (u64)t * 4;
+        u32 x[4], ix[4];
+        const u32 v = e < E ? src.load4(e, x, ix) : 0u;
+        u32 g = 0, q = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool ok = (v >> j) & 1u;
+            g += (ok && (ALL || x[j] > K)) ? 1u : 0u;
+            q += (!ALL && ok && x[j] == K) ? 1u : 0u;
+        }
+        u64 tot;
+        const u64 ex = block_excl_scan_u64<BR_NT>(((u64)q << 32) | g, L.scan, &tot);
+        u64 gb = gt_run + (u32)ex, qb = eq_run + (ex >> 32);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (!((v >> j) & 1u)) continue;
+            if (ALL || x[j] > K) {
+                emit(gb + min(qb, ties), x[j], ix[j]);
+                ++gb;
+            } else if (x[j] == K) {
+                if (qb < ties) emit(gb + qb, x[j], ix[j]);
+                ++qb;
+            }
+        }
+        gt_run += (u32)tot;
+        eq_run += tot >> 32;
+    }
+    return gt_run;
+}
+
+__global__ __launch_bounds__(BR_NT) void k_brows(const unsigned char* __restrict__ plan, const u32* __restrict__ keys,
+                                                 u32* __restrict__ wsq, const u32* __restrict__ deleg,
+                                                 u32* __restrict__ selws, u32* __restrict__ kth_out,
+                                                 u32* __restrict__ out_keys, i64* __restrict__ out_idx) {
+    const BatchHeader* h = (const BatchHeader*)plan;
+    const BatchRow* rows = (const BatchRow*)(plan + h->o_rows);
+    const u32* order = (const u32*)(plan + h->o_order);
+    const u64 nrows = h->nrows;
+    __shared__ RowLds L;
+    __shared__ __align__(16) u32 reg[REG_WORDS];
+    const int t = threadIdx.x;
+    for (;;) {
+        __syncthreads();
+        if (t == 0) L.misc[15] = atomicAdd(wsq, 1u);
+        __syncthreads();
+        const u32 q = L.misc[15];
+        if (q >= nrows) break;
+        const BatchRow* rp = rows + order[q];
+        const u64 n = rp->n, k = rp->k, out_off = rp->out_off;
+        const int a = rp->alpha, nb = rp->nb, f32 = rp->f32;
+        const u32* row = keys + rp->key_off;
+        u32* sel = selws + rp->sel_off;
+
+        // first top-k over the delegates and range selection
+        u32 T = 0;
+        u64 E = n;
+        if (a > 0) {
+            const SrcG dg{deleg + rp->deleg_off, rp->S};
+            u64 g = 0;
+            T = wg_kth(dg, rp->S, k, nb, true, reg, REG_WORDS, L, &g);
+            const u64 ties = k - g;
+            wg_ordered<false>(dg, rp->S, T, ties, L, [&](u64 p, u32, u32 id) {
+                if (p < k) sel[p] = id;
+            });
+            __syncthreads();
+            E = k << a;
+        }
+        // filtered concatenation, cached in LDS when it fits
+        const SrcV vs{row, n, sel, a, f32, T, E};
+        u32* ckey = reg;
+        u32* cidx = reg + CONCAT_CAP;
+        const u64 C = wg_ordered<true>(vs, E, 0u, 0, L, [&](u64 p, u32 x, u32 ix) {
+            if (p < CONCAT_CAP) {
+                ckey[p] = x;
+                cidx[p] = ix;
+            }
+        });
+        __syncthreads();
+        // second top-k and extraction
+        u64 gt = 0;
+        u32 K;
+        auto emit_out = [&](u64 p, u32 x, u32 ix) {
+            if (p < k) {
+                if (out_keys) out_keys[out_off + p] = f32 ? KeyF32::from_ord(x) : x;
+                if (out_idx) out_idx[out_off + p] = (i64)ix;
+            }
+        };
+        if (C <= CONCAT_CAP) {
+            const SrcL ls{ckey, cidx, (u32)C};
+            K = wg_kth(ls, C, k, nb, false, nullptr, 0, L, &gt);
+            wg_ordered<false>(ls, C, K, k - gt, L, emit_out);
+        } else {
+            K = wg_kth(vs, E, k, nb, true, reg, REG_WORDS, L, &gt);
+            wg_ordered<false>(vs, E, K, k - gt, L, emit_out);
+        }
+        if (t == 0 && kth_out) kth_out[order[q]] = f32 ? KeyF32::from_ord(K) : K;
+    }
+}
+
+// ---------------------------------------------------------------- tuning
+void default_tuning(drtopk_tuning* t) {
+    std::memset(t, DRTOPK_TUNE_DEFAULT, sizeof(*t));
+}
+
+// Built-in choice for a row: the reference's alpha policy (baseline+filter+beta+shuffle/
+// main.cu:83-89), 8-bit digits, 8192-key tiles.
+void choose(const drtopk_tuning& tu, u64 n, u64 k, int* alpha, int* nb, int* tile) {
+    const int ln = ceil_log2(n), lk = ceil_log2(k);
+    const u8 ta = tu.alpha[ln][lk], tn = tu.digit_bits[ln][lk], tt = tu.tile_log2[ln][lk];
+    int a = ta != DRTOPK_TUNE_DEFAULT ? (int)ta : std::max(0, (floor_log2(n) + 3 - lk) / 2);
+    int tl = tt != DRTOPK_TUNE_DEFAULT ? (int)tt : 13;
+    tl = std::min(MAX_TILE_LOG2, std::max(10, tl));
+    a = std::min(a, tl);
+    while (a > 0 && cdiv(n, 1ull << a) < k) --a;
+    *alpha = a;
+    *nb = (tn == 4 || tn == 8) ? (int)tn : 8;
+    *tile = tl;
+}
+
+}  // namespace
+}  // namespace drtopk
+
+using namespace drtopk;
+
+extern "C" {
+
+void drtopk_tuning_default(drtopk_tuning* t) {
+    if (t) default_tuning(t);
+}
+
+drtopk_status drtopk_batch_plan(const drtopk_row* rows, uint64_t nrows, uint64_t keys_len, uint64_t out_len,
+                                const drtopk_tuning* tuning, void* h_plan, size_t* plan_bytes, size_t* ws_bytes) {
+    clear_error();
+    if (!rows || !plan_bytes) return fail(DRTOPK_ERR_INVALID_ARG, "rows and plan_bytes must not be NULL");
+    if (nrows == 0 || nrows > (1ull << 24)) return fail(DRTOPK_ERR_INVALID_ARG, "nrows %llu not in [1, 2^24]", (unsigned long long)nrows);
+    drtopk_tuning tu;
+    if (tuning) tu = *tuning;
+    else default_tuning(&tu);
+    for (int i = 0; i < 32; ++i)
+        for (int j = 0; j < 32; ++j) {
+            const u8 d = tu.digit_bits[i][j], tl = tu.tile_log2[i][j], al = tu.alpha[i][j];
+            if (d != DRTOPK_TUNE_DEFAULT && d != 4 && d != 8)
+                return fail(DRTOPK_ERR_INVALID_ARG, "tuning digit_bits[%d][%d] = %d not 4 or 8", i, j, d);
+            if (tl != DRTOPK_TUNE_DEFAULT && (tl < 10 || tl > MAX_TILE_LOG2))
+                return fail(DRTOPK_ERR_INVALID_ARG, "tuning tile_log2[%d][%d] = %d not in [10, 15]", i, j, tl);
+            if (al != DRTOPK_TUNE_DEFAULT && al > 30)
+                return fail(DRTOPK_ERR_INVALID_ARG, "tuning alpha[%d][%d] = %d not in [0, 30]", i, j, al);
+        }
+    std::vector<BatchRow> br(nrows);
+    std::vector<u64> cost(nrows);
+    u64 nch = 0, dl = 0, sl = 0;
+    for (u64 r = 0; r < nrows; ++r) {
+        const drtopk_row& in = rows[r];
+        if (in.dtype != DRTOPK_U32 && in.dtype != DRTOPK_F32)
+            return fail(DRTOPK_ERR_INVALID_ARG, "row %llu: bad dtype %d", (unsigned long long)r, in.dtype);
+        if (in.reserved != 0) return fail(DRTOPK_ERR_INVALID_ARG, "row %llu: reserved must be 0", (unsigned long long)r);
+        if (in.n == 0 || in.n > (1ull << 31) - 1)
+            return fail(DRTOPK_ERR_INVALID_ARG, "row %llu: n = %llu not in [1, 2^31-1]", (unsigned long long)r,
+                        (unsigned long long)in.n);
+        if (in.k == 0 || in.k > in.n)
+            return fail(DRTOPK_ERR_INVALID_ARG, "row %llu: k = %llu not in [1, n = %llu]", (unsigned long long)r,
+                        (unsigned long long)in.k, (unsigned long long)in.n);
+        if (in.offset % 4 != 0)
+            return fail(DRTOPK_ERR_INVALID_ARG, "row %llu: offset %llu is not a multiple of 4 keys", (unsigned long long)r,
+                        (unsigned long long)in.offset);
+        if (in.offset > keys_len || in.n > keys_len - in.offset)
+            return fail(DRTOPK_ERR_INVALID_ARG, "row %llu: keys [%llu, +%llu) outside keys_len %llu", (unsigned long long)r,
+                        (unsigned long long)in.offset, (unsigned long long)in.n, (unsigned long long)keys_len);
+        if (in.out_offset > out_len || in.k > out_len - in.out_offset)
+            return fail(DRTOPK_ERR_INVALID_ARG, "row %llu: outputs [%llu, +%llu) outside out_len %llu", (unsigned long long)r,
+                        (unsigned long long)in.out_offset, (unsigned long long)in.k, (unsigned long long)out_len);
+        int a, nb, tl;
+        choose(tu, in.n, in.k, &a, &nb, &tl);
+        BatchRow& b = br[r];
+        std::memset(&b, 0, sizeof(b));
+        b.key_off = in.offset;
+        b.n = in.n;
+        b.k = in.k;
+        b.out_off = in.out_offset;
+        b.alpha = (u8)a;
+        b.f32 = in.dtype == DRTOPK_F32;
+        b.nb = (u8)nb;
+        b.tile_log2 = (u8)tl;
+        if (a > 0) {
+            const u64 S = cdiv(in.n, 1ull << a), ch = cdiv(in.n, 1ull << tl);
+            if (nch + ch > 0xFFFFFFFFull) return fail(DRTOPK_ERR_UNSUPPORTED, "batch has more than 2^32-1 tiles");
+            b.S = (u32)S;
+            b.chunk_begin = (u32)nch;
+            b.nchunks = (u32)ch;
+            b.deleg_off = dl;
+            b.sel_off = sl;
+            nch += ch;
+            dl += align_up(S, 4);
+            sl += align_up(in.k, 4);
+            cost[r] = 4 * S + (in.k << a);
+        } else {
+            cost[r] = 4 * in.n;
+        }
+    }
+    BatchHeader hd;
+    std::memset(&hd, 0, sizeof(hd));
+    hd.magic = PLAN_MAGIC;
+    hd.nrows = nrows;
+    hd.nchunks = nch;
+    u64 o = sizeof(BatchHeader);
+    hd.o_rows = o; o = align_up(o + sizeof(BatchRow) * nrows, 16);
+    hd.o_chunk_row = o; o = align_up(o + 4 * nch, 16);
+    hd.o_order = o; o = align_up(o + 4 * nrows, 16);
+    hd.plan_bytes = o;
+    u64 w = 256;
+    hd.o_ws_deleg = w; w = align_up(w + 4 * std::max<u64>(dl, 4), 256);
+    hd.o_ws_sel = w; w = align_up(w + 4 * std::max<u64>(sl, 4), 256);
+    hd.ws_bytes = w;
+    hd.keys_len = keys_len;
+    hd.out_len = out_len;
+    hd.sample_grid = (u32)std::max<u64>(1, std::min<u64>(nch, 4096));
+    hd.rows_grid = (u32)std::min<u64>(nrows, 512);
+    if (ws_bytes) *ws_bytes = (size_t)hd.ws_bytes;
+    if (!h_plan) {
+        *plan_bytes = (size_t)hd.plan_bytes;
+        return DRTOPK_OK;
+    }
+    if (*plan_bytes < hd.plan_bytes)
+        return fail(DRTOPK_ERR_WORKSPACE_TOO_SMALL, "plan buffer %zu bytes < %llu", *plan_bytes,
+                    (unsigned long long)hd.plan_bytes);
+    *plan_bytes = (size_t)hd.plan_bytes;
+    unsigned char* P = (unsigned char*)h_plan;
+    std::memset(P, 0, hd.plan_bytes);
+    std::memcpy(P, &hd, sizeof(hd));
+    std::memcpy(P + hd.o_rows, br.data(), sizeof(BatchRow) * nrows);
+    u32* crow = (u32*)(P + hd.o_chunk_row);
+    for (u64 r = 0; r < nrows; ++r)
+        for (u32 c = 0; c < br[r].nchunks; ++c) crow[br[r].chunk_begin + c] = (u32)r;
+    std::vector<u32> ord(nrows);
+    std::iota(ord.begin(), ord.end(), 0u);
+    std::stable_sort(ord.begin(), ord.end(), [&](u32 x, u32 y) { return cost[x] > cost[y]; });
+    std::memcpy(P + hd.o_order, ord.data(), 4 * nrows);
+    return DRTOPK_OK;
+}
+
+drtopk_status drtopk_select_batched(const void* h_plan, const void* d_plan, const void* d_keys, void* d_workspace,
+                                    size_t workspace_bytes, void* d_kth, void* d_topk_keys, int64_t* d_topk_idx,
+                                    void* stream) {
+    clear_error();
+    if (!h_plan || !d_plan) return fail(DRTOPK_ERR_INVALID_ARG, "h_plan and d_plan must not be NULL");
+    BatchHeader hd;
+    std::memcpy(&hd, h_plan, sizeof(hd));
+    if (hd.magic != PLAN_MAGIC) return fail(DRTOPK_ERR_INVALID_ARG, "h_plan is not a drtopk_batch_plan blob");
+    if (!d_keys) return fail(DRTOPK_ERR_INVALID_ARG, "d_keys is NULL");
+    if ((uintptr_t)d_keys % 16) return fail(DRTOPK_ERR_INVALID_ARG, "d_keys must be 16-byte aligned");
+    if ((uintptr_t)d_plan % 16) return fail(DRTOPK_ERR_INVALID_ARG, "d_plan must be 16-byte aligned");
+    if (!d_workspace || (uintptr_t)d_workspace % 256)
+        return fail(DRTOPK_ERR_INVALID_ARG, "d_workspace must be a 256-byte aligned device pointer");
+    if (workspace_bytes < hd.ws_bytes)
+        return fail(DRTOPK_ERR_WORKSPACE_TOO_SMALL, "workspace %zu bytes < %llu", workspace_bytes,
+                    (unsigned long long)hd.ws_bytes);
+    if ((d_topk_keys && (uintptr_t)d_topk_keys % 4) || (d_topk_idx && (uintptr_t)d_topk_idx % 8) ||
+        (d_kth && (uintptr_t)d_kth % 4))
+        return fail(DRTOPK_ERR_INVALID_ARG, "misaligned output pointer");
+    hipStream_t st = (hipStream_t)stream;
+    unsigned char* ws = (unsigned char*)d_workspace;
+    u32* wsq = (u32*)ws;
+    u32* deleg = (u32*)(ws + hd.o_ws_deleg);
+    u32* selws = (u32*)(ws + hd.o_ws_sel);
+    const unsigned char* P = (const unsigned char*)d_plan;
+    hipLaunchKernelGGL(k_bsample, dim3(hd.sample_grid), dim3(BS_NT), 0, st, P, (const u32*)d_keys, wsq, deleg);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_brows, dim3(hd.rows_grid), dim3(BR_NT), 0, st, P, (const u32*)d_keys, wsq,
+                       (const u32*)deleg, selws, (u32*)d_kth, (u32*)d_topk_keys, (i64*)d_topk_idx);
+    HIPCHK(hipGetLastError());
+    return DRTOPK_OK;
+}
+
+}  // extern "C"

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+typedef uint8_t u8;
 typedef uint32_t u32;
 typedef uint64_t u64;
 typedef int64_t i64;
@@ -120,6 +121,13 @@ __device__ __forceinline__ u32 wave_incl_scan_u32(u32 v) {
         u32 t = (u32)__shfl_up((int)v, o, WAVE);
         if (lane >= o) v += t;
     }
+    return v;
+}
+
+// u64 sum over the 64 lanes of a wave (result in every lane).
+__device__ __forceinline__ u64 wave_sum_u64(u64 v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += shfl_xor_u64(v, o);
     return v;
 }
 

@@ -126,6 +126,26 @@ struct ExtractArgs {
 constexpr u64 DIRECT_TILE = 16384;
 hipError_t launch_extract(const ExtractArgs& a, hipStream_t st);
 hipError_t launch_finalize(SelState* sel, Ctrl* ctrl, int f32, void* d_kth, u64 k, hipStream_t st);
+
+// bitonic.hip ---------------------------------------------------------------------------
+// Bitonic family: first top-k over the delegates, whole-subrange concatenation, second
+// top-k, index-ordered outputs.  alpha == 0: one top-k over V.
+struct BitonicArgs {
+    const u32* keys;
+    u64 n;
+    int alpha, f32;
+    u64 k, S;
+    const u32* dvals;  // delegates (alpha > 0)
+    Ctrl* ctrl;
+    u64* bufA;         // max(npad1, npad2) composites each
+    u64* bufB;
+    u64* dsel;         // k composites
+    u64 npad1, npad2;  // next powers of two of S and of k << alpha (or n)
+    void* out_keys;
+    i64* out_idx;
+    void* d_kth;
+};
+hipError_t launch_bitonic(const BitonicArgs& a, hipStream_t st);
 hipError_t launch_to_user(const u32* ord, u64 m, int f32, void* out, hipStream_t st);
 
 }  // namespace drtopk

@@ -147,6 +147,60 @@ drtopk_status drtopk_sample_delegates(drtopk_dtype dtype, const void* d_keys, ui
                                       int32_t alpha, int32_t beta, uint32_t flags, void* d_vals,
                                       uint32_t* d_pos, void* stream);
 
+/* ------------------------------------------------------------------------------------------
+ * Batched top-k: many independent rows of uneven length, each with its own k and key type, in
+ * two kernel launches.  The reference has no batched entry point; the nearest caller is its
+ * MultiGPU driver, which runs sample -> concat -> second top-k -> findTopk/FillTopk once per
+ * vector (MultiGPU/main.cu:366-424).  Per row the result is exactly drtopk_select's: k-th
+ * largest, top-k keys and indices (row-local, ascending, lowest index wins ties).
+ *
+ * Use: drtopk_batch_plan() on the host (validates the rows, picks alpha / digit width / tile per
+ * row from a tuning table, lays out the workspace) -> copy the plan blob to device memory once ->
+ * drtopk_select_batched() as often as wanted (stream-ordered, no host synchronisation, no
+ * allocation, no memset: it can be captured in a HIP graph).
+ * ------------------------------------------------------------------------------------------ */
+typedef struct drtopk_row {
+    uint64_t offset;      /* first key of the row in d_keys, in elements; a multiple of 4 */
+    uint64_t n;           /* keys in the row, 1 <= n <= 2^31-1 */
+    uint64_t k;           /* top rank, 1 <= k <= n */
+    uint64_t out_offset;  /* first of the row's k slots in d_topk_keys / d_topk_idx */
+    int32_t dtype;        /* drtopk_dtype of this row's keys */
+    int32_t reserved;     /* must be zero */
+} drtopk_row;
+
+#define DRTOPK_TUNE_DEFAULT 0xFFu
+/* Per-(ceil log2 n, ceil log2 k) bucket choices, as written by the project's autotuner
+ * (drtopksc_amd/autotune.py).  DRTOPK_TUNE_DEFAULT in a cell keeps the library's built-in
+ * choice.  alpha 0 = no delegates (the row is selected directly); the planner lowers an alpha
+ * that leaves fewer than k subranges or exceeds the tile.  digit_bits 4 or 8; tile_log2 10..15
+ * is log2 of the keys one workgroup of the delegate pass streams per work unit. */
+typedef struct drtopk_tuning {
+    uint8_t alpha[32][32];
+    uint8_t digit_bits[32][32];
+    uint8_t tile_log2[32][32];
+} drtopk_tuning;
+
+/* Plan a batch.  Host only.  Call with h_plan == NULL to get *plan_bytes; then again with a host
+ * buffer of that size.  keys_len / out_len are the element counts of d_keys / the output arrays
+ * (every row must fit).  tuning may be NULL (built-in table).  *ws_bytes receives the workspace
+ * size drtopk_select_batched needs. */
+drtopk_status drtopk_batch_plan(const drtopk_row* rows, uint64_t nrows, uint64_t keys_len,
+                                uint64_t out_len, const drtopk_tuning* tuning, void* h_plan,
+                                size_t* plan_bytes, size_t* ws_bytes);
+
+/* Run a planned batch.  h_plan is the host plan, d_plan a device copy of the same bytes.
+ *   d_keys       device keys of all rows (16-byte aligned); each row's keys are read as its dtype
+ *   d_kth        device array of nrows 32-bit keys (row r's k-th largest, in its dtype), or NULL
+ *   d_topk_keys  device array of out_len 32-bit keys, or NULL
+ *   d_topk_idx   device array of out_len int64 row-local indices, or NULL
+ * Calls that share a workspace must not overlap. */
+drtopk_status drtopk_select_batched(const void* h_plan, const void* d_plan, const void* d_keys,
+                                    void* d_workspace, size_t workspace_bytes, void* d_kth,
+                                    void* d_topk_keys, int64_t* d_topk_idx, void* stream);
+
+/* Built-in tuning table (what tuning == NULL uses). */
+void drtopk_tuning_default(drtopk_tuning* t);
+
 /* Thread-local message for the last error on this thread ("" if none). */
 const char* drtopk_last_error(void);
 

@@ -91,3 +91,13 @@ def test_select_rejects_bad_pointers(L):
 
 def test_version(L):
     assert b"gfx950" in L.drtopk_version()
+
+
+def test_bitonic_validation(L):
+    """The bitonic family needs a power-of-two k (sampleBitonic.cu:198-202)."""
+    assert ws(L, 2, 1 << 20, 1024)[0] == 0
+    assert ws(L, 2, 1 << 20, 1024, alpha=0)[0] == 0
+    assert ws(L, 2, 1 << 20, 1000)[0] == 1
+    assert "power-of-two" in L.drtopk_last_error().decode()
+    assert ws(L, 2, 1 << 20, 1 << 16, alpha=8)[0] == 1  # explicit alpha 8 leaves 4096 subranges < k
+    assert ws(L, 2, 1 << 20, 1 << 16)[0] == 0           # auto alpha is reduced until S > k

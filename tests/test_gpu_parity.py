@@ -189,3 +189,16 @@ def test_full_size_uniform(orc, dt):
         assert np.array_equal(tk.cpu().numpy().view(np.uint32), keys)
     del dev
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("name", ["uniform", "ties8", "lognormal", "f32normal", "ragged"])
+def test_pipeline_bitonic(orc, dt, name):
+    """Bitonic family (bitonic_firstTopk + bitonic, bitonic_com.cuh:610-728): k a power of two;
+    the LDS tile network (k <= 4096) and the global network (k > 4096)."""
+    data = {n: (v, f) for n, v, f in datasets(orc)}
+    v, f32 = data[name]
+    ok = orc.f32_to_ordered(v) if f32 else v
+    dev = to_dev_f32(v) if f32 else to_dev(v)
+    for k, alpha in [(1, -1), (2, 10), (16, -1), (128, 12), (1024, 8), (4096, 5), (8192, 6), (1 << 14, 5),
+                     (64, 0), (8192, 0)]:
+        compare(orc, dt, name, ok, dev, k, 2, orc.FLAGS_DEFAULT, alpha=alpha, f32=f32)
