@@ -2,12 +2,15 @@
 // (baseline+filter+beta+shuffle/main.cu:56-323; bucket_select/main.cu; bitonic/sampleBitonic.cu):
 //
 //   drtopk_topk <log2|V|> <k> <NBitsPerDigit> <beta> [--algo radix|bucket|bitonic] [--alpha A]
-//               [--flags F] [--seed S] [--check] [--buckets B]
+//               [--flags F] [--seed S] [--check] [--buckets B] [--reps R] [--topk] [--copy]
 //
 // Keys are uint32 drawn from std::mt19937(seed) (the reference seeds from std::random_device,
 // main.cu:75-76).  alpha defaults to the reference policy (main.cu:83).  --check sorts a host
 // copy and compares vec1[n-k] with the result, as the reference's Enabletest block does
 // (main.cu:281-296); the exit status is 0 on success, -1 (255) on a failed check or a bad argument.
+// --reps R repeats the timed call R times (phase times of the last one are printed); --topk also
+// writes the top-k keys and int64 indices, as the bench does.  --copy ends with one device-to-
+// device copy of the keys (a kernel of known traffic, to calibrate hardware byte counters).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,7 +26,7 @@
 int main(int argc, char** argv) {
     if (argc < 5) {
         std::printf("./exe num_element(log2) k NBitsPerDigit beta [--algo radix|bucket|bitonic] [--alpha A] "
-                    "[--flags F] [--seed S] [--check] [--buckets B]\n");
+                    "[--flags F] [--seed S] [--check] [--buckets B] [--reps R] [--topk] [--copy]\n");
         return -1;
     }
     const int num_pow = std::atoi(argv[1]);
@@ -33,7 +36,8 @@ int main(int argc, char** argv) {
     std::string algo = "radix";
     int alpha = -1, buckets = 256;
     unsigned flags = DRTOPK_FLAGS_DEFAULT, seed = 1;
-    bool check = false;
+    bool check = false, topk = false, copy = false;
+    int reps = 1;
     for (int i = 5; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--algo" && i + 1 < argc) algo = argv[++i];
@@ -42,6 +46,9 @@ int main(int argc, char** argv) {
         else if (a == "--seed" && i + 1 < argc) seed = (unsigned)std::strtoul(argv[++i], nullptr, 0);
         else if (a == "--buckets" && i + 1 < argc) buckets = std::atoi(argv[++i]);
         else if (a == "--check") check = true;
+        else if (a == "--topk") topk = true;
+        else if (a == "--copy") copy = true;
+        else if (a == "--reps" && i + 1 < argc) reps = std::max(1, std::atoi(argv[++i]));
         else { std::printf("wrong input: %s\n", a.c_str()); return -1; }
     }
     if (num_pow < 1 || num_pow > 30 || k < 1 || k > (1ll << num_pow)) { std::printf("wrong input\n"); return -1; }
@@ -69,9 +76,15 @@ int main(int argc, char** argv) {
     void* ws = nullptr;
     unsigned* kth_d = nullptr;
     if (hipMalloc(&ws, bytes) != hipSuccess || hipMalloc(&kth_d, 4) != hipSuccess) { std::printf("hipMalloc failed\n"); return -1; }
+    unsigned* keys_d = nullptr;
+    int64_t* idx_d = nullptr;
+    if (topk && (hipMalloc(&keys_d, 4 * (size_t)k) != hipSuccess || hipMalloc(&idx_d, 8 * (size_t)k) != hipSuccess)) {
+        std::printf("hipMalloc failed\n");
+        return -1;
+    }
     drtopk_stats st;
-    for (int rep = 0; rep < 2; ++rep) {  // first call warms up the code objects
-        s = drtopk_select(a, DRTOPK_U32, vec_d, n, (uint64_t)k, &p, ws, bytes, kth_d, nullptr, nullptr, &st, nullptr);
+    for (int rep = 0; rep < 1 + reps; ++rep) {  // the first call warms up the code objects
+        s = drtopk_select(a, DRTOPK_U32, vec_d, n, (uint64_t)k, &p, ws, bytes, kth_d, keys_d, idx_d, &st, nullptr);
         if (s != DRTOPK_OK) { std::printf("error: %s\n", drtopk_last_error()); return -1; }
     }
     unsigned TopKElement = 0;
@@ -93,6 +106,17 @@ int main(int argc, char** argv) {
         else { std::printf("Not Success!\n"); rc = -1; }
         std::printf("Required value%u\n", vec[n - k]);
     }
+    if (copy) {
+        unsigned* dup = nullptr;
+        if (hipMalloc(&dup, n * 4) != hipSuccess || hipMemcpy(dup, vec_d, n * 4, hipMemcpyDeviceToDevice) != hipSuccess) {
+            std::printf("copy failed\n");
+            return -1;
+        }
+        std::printf("copied %llu bytes device to device\n", (unsigned long long)(n * 4));
+        (void)hipFree(dup);
+    }
+    if (keys_d) (void)hipFree(keys_d);
+    if (idx_d) (void)hipFree(idx_d);
     (void)hipFree(ws);
     (void)hipFree(kth_d);
     (void)hipFree(vec_d);

@@ -333,7 +333,72 @@ u64 gather_nseg(u64 V) {
 }
 
 // ---------------------------------------------------------------- beta-path range selection + concatenation
-constexpr int BSPT = 4;  // subranges per thread (1024 per chunk)
+constexpr int BSPT = 8;  // subranges per thread (2048 per chunk)
+
+// Per chunk of 2048 subranges: (1) each thread classifies its 8 subranges from their
+// delegates and packed offsets (prefetched one chunk ahead) and the selected ones are listed in LDS in id order;
+// (2) the listed subranges' keys are copied to LDS with 16-byte loads (re-read from global
+// memory instead when they exceed the buffer) and their kept keys counted, 2^alpha/4 lanes
+// per subrange; (3) a block scan places every subrange's output (selected: its kept keys,
+// lonely: its delegates, in index order); (4) each listed key goes to its slot by a prefix
+// sum over its subrange's lanes.  Survivors are staged in LDS and leave with coalesced stores.
+constexpr u32 BSTAGE = 2048;  // staged survivors per chunk (more: direct stores)
+constexpr u32 KBUF = 2048;    // keys of listed subranges held in LDS
+
+template <int BETA>
+__device__ __forceinline__ void load_deleg(const u32* __restrict__ dvals, u64 s0, u64 S, u32 (&d)[BSPT][BETA]) {
+    if (s0 + BSPT <= S) {
+        const u32x4* p = (const u32x4*)(dvals + s0 * BETA);  // BSPT*BETA words, 16-byte aligned
+        u32 w[BSPT * BETA];
+#pragma unroll
+        for (int q = 0; q < BSPT * BETA / 4; ++q) {
+            const u32x4 r = p[q];
+            w[4 * q] = r.x; w[4 * q + 1] = r.y; w[4 * q + 2] = r.z; w[4 * q + 3] = r.w;
+        }
+#pragma unroll
+        for (int j = 0; j < BSPT; ++j)
+#pragma unroll
+            for (int b = 0; b < BETA; ++b) d[j][b] = w[j * BETA + b];
+    } else {
+#pragma unroll
+        for (int j = 0; j < BSPT; ++j)
+#pragma unroll
+            for (int b = 0; b < BETA; ++b) d[j][b] = s0 + j < S ? dvals[(s0 + j) * BETA + b] : 0u;
+    }
+}
+
+// Packed in-subrange offsets of the first beta-1 delegates of the thread's BSPT subranges
+// (u8 / u16 / u32 per subrange for beta 2 / 3 / 4).
+template <int BETA>
+__device__ __forceinline__ void load_doff(const void* __restrict__ doff, u64 s0, u64 S, u32 (&pk)[BSPT]) {
+    if (s0 + BSPT <= S) {
+        if (BETA == 2) {
+            const u64 q = *(const u64*)((const uint8_t*)doff + s0);
+#pragma unroll
+            for (int j = 0; j < BSPT; ++j) pk[j] = (u32)(q >> (8 * j)) & 0xFFu;
+        } else if (BETA == 3) {
+            const u32x4 q = *(const u32x4*)((const uint16_t*)doff + s0);
+#pragma unroll
+            for (int j = 0; j < BSPT; ++j) pk[j] = (q[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+        } else {
+            const u32x4 q0 = *(const u32x4*)((const u32*)doff + s0);
+            const u32x4 q1 = *(const u32x4*)((const u32*)doff + s0 + 4);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { pk[j] = q0[j]; pk[4 + j] = q1[j]; }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < BSPT; ++j) {
+            const u64 s = s0 + j;
+            pk[j] = 0;
+            if (s < S) {
+                if (BETA == 2) pk[j] = ((const uint8_t*)doff)[s];
+                else if (BETA == 3) pk[j] = ((const uint16_t*)doff)[s];
+                else pk[j] = ((const u32*)doff)[s];
+            }
+        }
+    }
+}
 
 template <class K, int BETA, bool POS, bool HIST2>
 __global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys, u64 n, int alpha, int filter,
@@ -344,8 +409,11 @@ __global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys
     __shared__ u64 scratch[WG / WAVE + 1];
     __shared__ u32 lh[HIST2 ? 256 : 1];
     __shared__ u64 red[3][WG / WAVE];
-    __shared__ u32 sk[STAGE];
-    __shared__ u32 si[POS ? STAGE : 1];
+    __shared__ u32 sk[BSTAGE];
+    __shared__ u32 si[POS ? BSTAGE : 1];
+    __shared__ uint16_t slist[BETA_TILE_SUBS];  // selected subranges of the chunk (chunk-local ids), ascending
+    __shared__ u32 spos[BETA_TILE_SUBS];   // their kept-key counts, then their first output slot
+    __shared__ __align__(16) u32 kbuf[KBUF];
     h2_init<HIST2>(lh, bins2);
     const u32 thr = ctrl->first.result;
     const u64 L = 1ull << alpha;
@@ -353,76 +421,153 @@ __global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys
     const u64 nchunks = (S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS;
     const u64 nseg = (nchunks + cps - 1) / cps;
     const u64 segcap = cps * (BETA_TILE_SUBS << alpha);
-    const bool vec_keys = alpha >= 2;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int G = alpha >= 2 ? (1 << (alpha - 2)) : 1;  // lanes per listed subrange (alpha <= 8)
+    const int gl = lane & (G - 1), grp = lane / G, per_wave = WAVE / G;
     u64 nsel = 0, nlon = 0, nlel = 0;
-    auto count_sel = [&](u64 base, u64 len) -> u32 {
-        if (!filter) return (u32)len;
-        u32 c = 0;
-        if (vec_keys && len == L) {
-            const u32x4* p = (const u32x4*)(keys + base);
-            for (u64 q = 0; q < (L >> 2); ++q) {
-                const u32x4 r = p[q];
-                c += (K::to_ord(r.x) >= thr) + (K::to_ord(r.y) >= thr) + (K::to_ord(r.z) >= thr) +
-                     (K::to_ord(r.w) >= thr);
-            }
-        } else {
-            for (u64 i = base; i < base + len; ++i) c += K::to_ord(keys[i]) >= thr;
-        }
-        return c;
-    };
     for (u64 sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
         u64 run = 0;
-        for (u64 ch = sg * cps; ch < std::min<u64>((sg + 1) * cps, nchunks); ++ch) {
-            const u64 s0 = ch * BETA_TILE_SUBS + (u64)threadIdx.x * BSPT;
-            u32 d[BSPT][BETA];
-            if (s0 + BSPT <= S) {
-                const u32x4* p = (const u32x4*)(dvals + s0 * BETA);  // 4*BETA words, 16-byte aligned
-                u32 w[BSPT * BETA];
-#pragma unroll
-                for (int q = 0; q < BETA; ++q) {
-                    const u32x4 r = p[q];
-                    w[4 * q] = r.x; w[4 * q + 1] = r.y; w[4 * q + 2] = r.z; w[4 * q + 3] = r.w;
-                }
-#pragma unroll
-                for (int j = 0; j < BSPT; ++j)
-#pragma unroll
-                    for (int b = 0; b < BETA; ++b) d[j][b] = w[j * BETA + b];
-            } else {
-#pragma unroll
-                for (int j = 0; j < BSPT; ++j)
-#pragma unroll
-                    for (int b = 0; b < BETA; ++b) d[j][b] = s0 + j < S ? dvals[(s0 + j) * BETA + b] : 0u;
+        const u64 ch_end = std::min<u64>((sg + 1) * cps, nchunks);
+        u32 d[BSPT][BETA], pk[BSPT];
+        load_deleg<BETA>(dvals, sg * cps * BETA_TILE_SUBS + (u64)threadIdx.x * BSPT, S, d);
+        if (POS) load_doff<BETA>(doff, sg * cps * BETA_TILE_SUBS + (u64)threadIdx.x * BSPT, S, pk);
+        for (u64 ch = sg * cps; ch < ch_end; ++ch) {
+            const u64 c0 = ch * BETA_TILE_SUBS;
+            const u64 s0 = c0 + (u64)threadIdx.x * BSPT;
+            u32 dn[BSPT][BETA], pkn[BSPT];
+            const bool more = ch + 1 < ch_end;
+            if (more) {
+                load_deleg<BETA>(dvals, s0 + BETA_TILE_SUBS, S, dn);
+                if (POS) load_doff<BETA>(doff, s0 + BETA_TILE_SUBS, S, pkn);
             }
             int kind[BSPT];  // 0 none, 1 selected (its keys), 2 lonely (its delegates)
-            u32 c = 0;
+            u32 mj[BSPT];    // delegates of the subrange (min(beta, its length))
+            u32 ns_t = 0;
 #pragma unroll
             for (int j = 0; j < BSPT; ++j) {
                 const u64 s = s0 + j;
                 kind[j] = 0;
+                mj[j] = 0;
                 if (s >= S || d[j][0] < thr) continue;
-                const u64 base = s << alpha;
-                const u64 len = std::min<u64>(L, n - base);
+                const u64 len = std::min<u64>(L, n - (s << alpha));
                 const int m = (int)std::min<u64>(BETA, len);
                 bool all = true;
 #pragma unroll
                 for (int b = 0; b < BETA; ++b) all = all && (b >= m || d[j][b] >= thr);
+                mj[j] = (u32)m;
                 if (all) {
                     kind[j] = 1;
                     ++nsel;
-                    c += count_sel(base, len);
+                    ++ns_t;
                 } else {
                     kind[j] = 2;
                     ++nlon;
                     nlel += (u64)m;
-                    c += (u32)m;
+                }
+            }
+            // (1) list the selected subranges
+            u64 tot_s;
+            const u64 ex_s = block_excl_scan_u64<WG>((u64)ns_t, scratch, &tot_s);
+            {
+                u32 q = (u32)ex_s;
+#pragma unroll
+                for (int j = 0; j < BSPT; ++j)
+                    if (kind[j] == 1) slist[q++] = (uint16_t)(threadIdx.x * BSPT + j);
+            }
+            __syncthreads();
+            const u32 ns = (u32)tot_s;
+            const bool inl = alpha >= 2 && ((u64)ns << alpha) <= KBUF;  // uniform
+            if (inl) {
+                const u32 ngran = (ns << alpha) >> 2;
+                const int lg = alpha - 2;
+                for (u32 x0 = 0; x0 < ngran; x0 += 2 * WG) {
+                    u32x4 r[2];
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const u32 x = x0 + u * WG + threadIdx.x;
+                        r[u] = u32x4{0, 0, 0, 0};
+                        if (x < ngran) {
+                            const u64 e = ((c0 + slist[x >> lg]) << alpha) + 4ull * (x & ((1u << lg) - 1));
+                            if (e + 4 <= n) r[u] = *(const u32x4*)(keys + e);
+                            else
+                                for (int jj = 0; jj < 4; ++jj)
+                                    if (e + jj < n) r[u][jj] = keys[e + jj];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const u32 x = x0 + u * WG + threadIdx.x;
+                        if (x < ngran) {
+                            u32x4 v = r[u];
+                            v.x = K::to_ord(v.x); v.y = K::to_ord(v.y); v.z = K::to_ord(v.z); v.w = K::to_ord(v.w);
+                            *(u32x4*)(kbuf + 4 * x) = v;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            // ordered keys and valid mask of granule gl (4 keys) of listed subrange i
+            auto fetch4 = [&](u32 i, u32* kv, u64* e_out) -> u32 {
+                const u64 e = ((c0 + slist[i]) << alpha) + 4 * (u64)gl;
+                *e_out = e;
+                u32 okm = 0;
+                if (inl) {
+                    const u32x4 v = *(const u32x4*)(kbuf + ((u64)i << alpha) + 4 * gl);
+                    kv[0] = v.x; kv[1] = v.y; kv[2] = v.z; kv[3] = v.w;
+                    for (int jj = 0; jj < 4; ++jj) okm |= (e + jj < n ? 1u : 0u) << jj;
+                } else if (e + 4 <= n) {
+                    const u32x4 r = *(const u32x4*)(keys + e);
+                    kv[0] = K::to_ord(r.x); kv[1] = K::to_ord(r.y); kv[2] = K::to_ord(r.z); kv[3] = K::to_ord(r.w);
+                    okm = 0xF;
+                } else {
+                    for (int jj = 0; jj < 4; ++jj) {
+                        kv[jj] = e + jj < n ? K::to_ord(keys[e + jj]) : 0u;
+                        okm |= (e + jj < n ? 1u : 0u) << jj;
+                    }
+                }
+                if (filter)
+                    for (int jj = 0; jj < 4; ++jj)
+                        if (kv[jj] < thr) okm &= ~(1u << jj);
+                return okm;
+            };
+            // (2) kept keys of each listed subrange
+            if (alpha >= 2) {
+                for (u32 i0 = (u32)wv * per_wave; i0 < ns; i0 += (WG / WAVE) * per_wave) {  // uniform per wave
+                    const u32 i = i0 + grp;
+                    u32 c = 0;
+                    if (i < ns) {
+                        u32 kv[4];
+                        u64 e;
+                        c = (u32)__popc(fetch4(i, kv, &e));
+                    }
+                    for (int o = 1; o < G; o <<= 1) c += (u32)__shfl_xor((int)c, o, WAVE);
+                    if (gl == 0 && i < ns) spos[i] = c;
+                }
+            } else {
+                for (u32 i = threadIdx.x; i < ns; i += WG) {
+                    const u64 base = (c0 + slist[i]) << alpha;
+                    const u64 len = std::min<u64>(L, n - base);
+                    u32 c = 0;
+                    for (u64 e = base; e < base + len; ++e) c += (!filter || K::to_ord(keys[e]) >= thr) ? 1u : 0u;
+                    spos[i] = c;
+                }
+            }
+            __syncthreads();
+            // (3) output slots
+            u32 c = 0;
+            {
+                u32 q = (u32)ex_s;
+#pragma unroll
+                for (int j = 0; j < BSPT; ++j) {
+                    if (kind[j] == 1) c += spos[q++];
+                    else if (kind[j] == 2) c += mj[j];
                 }
             }
             u64 tot;
             const u64 ex = block_excl_scan_u64<WG>((u64)c, scratch, &tot);
-            const bool staged = tot <= STAGE;  // uniform
+            const bool staged = tot <= BSTAGE;  // uniform
             const u64 gbase = sg * segcap + run;
-            u64 slot = ex;
-            auto put = [&](u32 o, u32 gidx) {
+            auto put = [&](u64 slot, u32 o, u32 gidx) {
                 if (staged) {
                     sk[slot] = o;
                     if (POS) si[slot] = gidx;
@@ -431,57 +576,93 @@ __global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys
                     if (POS) sidx[gbase + slot] = gidx;
                     if (HIST2) atomicAdd(&lh[o >> hshift2], 1u);
                 }
-                ++slot;
             };
+            {
+                u64 slot = ex;
+                u32 q = (u32)ex_s;
 #pragma unroll
-            for (int j = 0; j < BSPT; ++j) {
-                if (kind[j] == 0) continue;
-                const u64 s = s0 + j;
-                const u64 base = s << alpha;
-                const u64 len = std::min<u64>(L, n - base);
-                if (kind[j] == 1) {
-                    for (u64 i = base; i < base + len; ++i) {
-                        const u32 o = K::to_ord(keys[i]);
-                        if (!filter || o >= thr) put(o, (u32)i);
+                for (int j = 0; j < BSPT; ++j) {
+                    if (kind[j] == 1) {
+                        const u32 cj = spos[q];
+                        spos[q] = (u32)slot;
+                        slot += cj;
+                        ++q;
+                    } else if (kind[j] == 2) {
+                        const u64 s = s0 + j;
+                        const u64 base = s << alpha;
+                        const int m = (int)mj[j];
+                        u32 dv[BETA], p[BETA];
+                        const u32 packed = POS ? pk[j] : 0u;
+#pragma unroll
+                        for (int b = 0; b < BETA; ++b) {
+                            dv[b] = d[j][b];
+                            const bool ge = b < m && dv[b] >= thr;  // only these can reach the output
+                            p[b] = (POS && ge && b < BETA - 1) ? (u32)(base + ((packed >> (8 * b)) & 0xFFu))
+                                                               : 0xFFFFFFFFu;
+                        }
+                        // delegates >= threshold in index order, then the rest (never selected later)
+#pragma unroll
+                        for (int x = 0; x < BETA; ++x)
+#pragma unroll
+                            for (int y = 0; y + 1 < BETA - x; ++y)
+                                if (p[y + 1] < p[y]) {
+                                    const u32 tp = p[y]; p[y] = p[y + 1]; p[y + 1] = tp;
+                                    const u32 td = dv[y]; dv[y] = dv[y + 1]; dv[y + 1] = td;
+                                }
+#pragma unroll
+                        for (int b = 0; b < BETA; ++b)
+                            if (b < m) put(slot++, dv[b], p[b]);
                     }
-                } else {
-                    const int m = (int)std::min<u64>(BETA, len);
-                    u32 dv[BETA], p[BETA];
-                    u32 packed = 0;
-                    if (POS) {
-                        if (BETA == 2) packed = ((const uint8_t*)doff)[s];
-                        else if (BETA == 3) packed = ((const uint16_t*)doff)[s];
-                        else packed = ((const u32*)doff)[s];
+                }
+            }
+            __syncthreads();
+            // (4) keys of the listed subranges to their slots
+            if (alpha >= 2) {
+                for (u32 i0 = (u32)wv * per_wave; i0 < ns; i0 += (WG / WAVE) * per_wave) {
+                    const u32 i = i0 + grp;
+                    u32 kv[4] = {0, 0, 0, 0}, okm = 0;
+                    u64 e = 0;
+                    if (i < ns) okm = fetch4(i, kv, &e);
+                    const u32 c4 = (u32)__popc(okm);
+                    u32 incl = c4;
+                    for (int o = 1; o < G; o <<= 1) {
+                        const u32 tt = (u32)__shfl_up((int)incl, o, WAVE);
+                        if (gl >= o) incl += tt;
                     }
-#pragma unroll
-                    for (int b = 0; b < BETA; ++b) {
-                        dv[b] = d[j][b];
-                        const bool ge = b < m && dv[b] >= thr;  // only these can reach the output
-                        p[b] = (POS && ge && b < BETA - 1) ? (u32)(base + ((packed >> (8 * b)) & 0xFFu)) : 0xFFFFFFFFu;
+                    if (i < ns) {
+                        u64 slot = spos[i] + (incl - c4);
+                        for (int jj = 0; jj < 4; ++jj)
+                            if ((okm >> jj) & 1u) put(slot++, kv[jj], (u32)(e + jj));
                     }
-                    // delegates >= threshold in index order, then the rest (never selected later)
-#pragma unroll
-                    for (int x = 0; x < BETA; ++x)
-#pragma unroll
-                        for (int y = 0; y + 1 < BETA - x; ++y)
-                            if (p[y + 1] < p[y]) {
-                                const u32 tp = p[y]; p[y] = p[y + 1]; p[y + 1] = tp;
-                                const u32 td = dv[y]; dv[y] = dv[y + 1]; dv[y + 1] = td;
-                            }
-#pragma unroll
-                    for (int b = 0; b < BETA; ++b)
-                        if (b < m) put(dv[b], p[b]);
+                }
+            } else {
+                for (u32 i = threadIdx.x; i < ns; i += WG) {
+                    const u64 base = (c0 + slist[i]) << alpha;
+                    const u64 len = std::min<u64>(L, n - base);
+                    u64 slot = spos[i];
+                    for (u64 e = base; e < base + len; ++e) {
+                        const u32 o = K::to_ord(keys[e]);
+                        if (!filter || o >= thr) put(slot++, o, (u32)e);
+                    }
                 }
             }
             __syncthreads();
             if (staged) flush_stage<POS, HIST2>(sk, si, (u32)tot, skeys, sidx, gbase, lh, hshift2);
             __syncthreads();
             run += tot;
+            if (more) {
+#pragma unroll
+                for (int j = 0; j < BSPT; ++j) {
+#pragma unroll
+                    for (int b = 0; b < BETA; ++b) d[j][b] = dn[j][b];
+                    pk[j] = pkn[j];
+                }
+            }
         }
         if (threadIdx.x == 0) cnt[sg] = (u32)run;
     }
     {
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int w = threadIdx.x >> 6;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             nsel += shfl_xor_u64(nsel, o);

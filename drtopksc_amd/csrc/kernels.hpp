@@ -32,6 +32,9 @@ struct SampleArgs {
 };
 hipError_t launch_sample(const SampleArgs& a, hipStream_t st);
 
+// Compute units of the current device (cached per device).
+unsigned device_cus();
+
 // Selection machinery -------------------------------------------------------------------
 // mode 0: radix digits (prefix match + digit); mode 1: bucket levels (range + linear map)
 hipError_t launch_init(Ctrl* ctrl, const Ctrl& init, u32* zero_begin, u64 zero_words, hipStream_t st);
@@ -102,7 +105,7 @@ struct BetaGatherArgs {
     u32* ghist2;
     int hshift2, bins2;
 };
-constexpr u64 BETA_TILE_SUBS = 1024;
+constexpr u64 BETA_TILE_SUBS = 2048;
 hipError_t launch_beta_gather(const BetaGatherArgs& a, hipStream_t st);
 u64 beta_segcap(u64 S, int alpha);
 u64 beta_nseg(u64 S);

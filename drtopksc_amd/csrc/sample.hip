@@ -363,73 +363,99 @@ __global__ __launch_bounds__(WG) void k_sample_waves(const u32* __restrict__ key
 }
 
 // ---------------------------------------------------------------- LDS staged, alpha 1..5
+// A tile is 8192 keys (2^(13-alpha) subranges, 2^(5-alpha) per thread).  The next tile's
+// eight 16-byte loads per lane are issued before the current tile is reduced from LDS, so a
+// workgroup always has 32 KB of reads in flight.
+constexpr int STAGE_Q = 8;                     // uint4 per thread per tile
+constexpr int STAGE_KEYS = STAGE_Q * WG * 4;   // 8192
+
+template <class K>
+__device__ __forceinline__ void stage_fetch(const u32* __restrict__ keys, u64 n, u64 k0, bool full, u32x4 (&r)[STAGE_Q]) {
+    if (full) {
+        const u32x4* p = (const u32x4*)(keys + k0);
+#pragma unroll
+        for (int u = 0; u < STAGE_Q; ++u) r[u] = load4(p + u * WG + threadIdx.x);
+    } else {
+#pragma unroll
+        for (int u = 0; u < STAGE_Q; ++u) {
+            const u64 i = k0 + 4ull * (u * WG + threadIdx.x);
+            u32x4 v = {0, 0, 0, 0};
+            if (i + 0 < n) v.x = keys[i + 0];
+            if (i + 1 < n) v.y = keys[i + 1];
+            if (i + 2 < n) v.z = keys[i + 2];
+            if (i + 3 < n) v.w = keys[i + 3];
+            r[u] = v;
+        }
+    }
+}
+
 template <class K, int BETA, int POSM, int FUSE, class LT>
 __global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ keys, u64 n, int alpha, int vec_ok,
                                                       u64 ntiles, u32* __restrict__ dvals, void* __restrict__ dpos,
                                                       u32* __restrict__ ghist, int hshift, int bins, Ctrl* ctrl) {
-    // 256 subranges of up to 32 keys; one pad uint4 after every 8 uint4 spreads LDS banks
-    __shared__ u32x4 tile[WG * 8 + WG];
+    // one pad uint4 after every 8 uint4 spreads the threads' subrange reads over the LDS banks
+    __shared__ u32x4 tile[STAGE_Q * WG + STAGE_Q * WG / 8];
     __shared__ u32 lhist[FUSE == 1 ? 256 : 1];
     __shared__ u32 red[16];
     fuse_init<FUSE>(lhist, bins);
     u32 mn = 0xFFFFFFFFu, mx = 0u;
     const u64 L = 1ull << alpha;
     const u64 S = (n + L - 1) >> alpha;
-    const int qpt = alpha >= 2 ? (int)(L >> 2) : 1;  // uint4 per thread
-    for (u64 tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
-        const u64 s0 = tt * WG;
-        const u64 k0 = s0 << alpha;
-        const int tile_q = (WG << alpha) >> 2;  // uint4 in the tile
-        const bool full = vec_ok && k0 + ((u64)WG << alpha) <= n && alpha >= 2;
-        if (full) {
-            const u32x4* p = (const u32x4*)(keys + k0);
-            for (int q = threadIdx.x; q < tile_q; q += WG) {
-                u32x4 r = load4(p + q);
-                r.x = K::to_ord(r.x); r.y = K::to_ord(r.y); r.z = K::to_ord(r.z); r.w = K::to_ord(r.w);
-                tile[q + (q >> 3)] = r;
-            }
-        } else {
-            for (int q = threadIdx.x; q < ((WG << alpha) + 3) / 4; q += WG) {
-                u32x4 r = {0, 0, 0, 0};
-                const u64 i = k0 + 4ull * q;
-                if (i + 0 < n) r.x = K::to_ord(keys[i + 0]);
-                if (i + 1 < n) r.y = K::to_ord(keys[i + 1]);
-                if (i + 2 < n) r.z = K::to_ord(keys[i + 2]);
-                if (i + 3 < n) r.w = K::to_ord(keys[i + 3]);
-                tile[q + (q >> 3)] = r;
-            }
+    const int spt = STAGE_KEYS >> alpha;  // subranges per tile
+    const int qps = alpha >= 2 ? (int)(L >> 2) : 1;  // uint4 per subrange
+    u32x4 r[STAGE_Q];
+    u64 tt = blockIdx.x;
+    if (tt < ntiles) {
+        const u64 k0 = tt * STAGE_KEYS;
+        stage_fetch<K>(keys, n, k0, vec_ok && k0 + STAGE_KEYS <= n, r);
+    }
+    for (; tt < ntiles; tt += gridDim.x) {
+        const u64 k0 = tt * STAGE_KEYS;
+#pragma unroll
+        for (int u = 0; u < STAGE_Q; ++u) {
+            const int q = u * WG + threadIdx.x;
+            u32x4 v = r[u];
+            v.x = K::to_ord(v.x); v.y = K::to_ord(v.y); v.z = K::to_ord(v.z); v.w = K::to_ord(v.w);
+            tile[q + (q >> 3)] = v;
+        }
+        const u64 nt = tt + gridDim.x;
+        if (nt < ntiles) {
+            const u64 k1 = nt * STAGE_KEYS;
+            stage_fetch<K>(keys, n, k1, vec_ok && k1 + STAGE_KEYS <= n, r);
         }
         __syncthreads();
-        const u64 s = s0 + threadIdx.x;
-        LT t;
-        t.clear();
-        const u64 kb = k0 + ((u64)threadIdx.x << alpha);
-        if (alpha >= 2) {
-            for (int i = 0; i < qpt; ++i) {
-                const int q = threadIdx.x * qpt + i;
-                const u32x4 r = tile[q + (q >> 3)];
-                const u64 i0 = kb + 4ull * i;
-                if (i0 + 3 < n) {
-                    t.insert(r.x, i0); t.insert(r.y, i0 + 1);
-                    t.insert(r.z, i0 + 2); t.insert(r.w, i0 + 3);
-                } else {
-                    if (i0 + 0 < n) t.insert(r.x, i0);
-                    if (i0 + 1 < n) t.insert(r.y, i0 + 1);
-                    if (i0 + 2 < n) t.insert(r.z, i0 + 2);
+        const u64 s0 = k0 >> alpha;
+        for (int j = threadIdx.x; j < spt; j += WG) {  // uniform trip count: spt is a multiple of WG
+            const u64 s = s0 + j;
+            const u64 kb = k0 + ((u64)j << alpha);
+            LT t;
+            t.clear();
+            if (alpha >= 2) {
+                for (int i = 0; i < qps; ++i) {
+                    const int q = j * qps + i;
+                    const u32x4 v = tile[q + (q >> 3)];
+                    const u64 i0 = kb + 4ull * i;
+                    if (i0 + 3 < n) {
+                        t.insert(v.x, i0); t.insert(v.y, i0 + 1);
+                        t.insert(v.z, i0 + 2); t.insert(v.w, i0 + 3);
+                    } else {
+                        if (i0 + 0 < n) t.insert(v.x, i0);
+                        if (i0 + 1 < n) t.insert(v.y, i0 + 1);
+                        if (i0 + 2 < n) t.insert(v.z, i0 + 2);
+                    }
                 }
+            } else {
+                // alpha == 1: two keys per subrange
+                const int q = j >> 1;
+                const u32x4 v = tile[q + (q >> 3)];
+                const u32 a = (j & 1) == 0 ? v.x : v.z, b = (j & 1) == 0 ? v.y : v.w;
+                if (kb < n) t.insert(a, kb);
+                if (kb + 1 < n) t.insert(b, kb + 1);
             }
-        } else {
-            // alpha == 1: two keys per subrange
-            const int j = threadIdx.x * 2;
-            const int q = j >> 2;
-            const u32x4 r = tile[q + (q >> 3)];
-            const u32 a = (j & 3) == 0 ? r.x : r.z, b = (j & 3) == 0 ? r.y : r.w;
-            if (kb < n) t.insert(a, kb);
-            if (kb + 1 < n) t.insert(b, kb + 1);
+            const bool leader = s < S;
+            if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos);
+            fuse_add<BETA, FUSE, LT>(t, leader, lhist, hshift, &mn, &mx);
         }
-        const bool leader = s < S;
-        if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos);
-        fuse_add<BETA, FUSE, LT>(t, leader, lhist, hshift, &mn, &mx);
         __syncthreads();
     }
     fuse_flush<FUSE>(lhist, ghist, bins, mn, mx, ctrl, red);
@@ -445,8 +471,9 @@ static hipError_t launch_sample_t(const SampleArgs& a, hipStream_t st) {
     const u64 S = (a.n + L - 1) >> a.alpha;
     const int vec_ok = ((uintptr_t)a.keys & 15) == 0;
     if (a.staged && a.alpha <= 5) {
-        const u64 ntiles = (S + WG - 1) / WG;
-        const unsigned grid = (unsigned)std::min<u64>(ntiles, SAMPLE_GRID);
+        // exactly the resident workgroups (LDS allows 4 per CU): every one keeps its pipeline full
+        const u64 ntiles = (a.n + STAGE_KEYS - 1) / STAGE_KEYS;
+        const unsigned grid = (unsigned)std::min<u64>(ntiles, 4ull * device_cus());
         hipLaunchKernelGGL((k_sample_staged<K, BETA, POSM, FUSE, LT>), dim3(grid), dim3(WG), 0, st, a.keys, a.n,
                            a.alpha, vec_ok, ntiles, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl);
     } else if (a.alpha <= 8) {

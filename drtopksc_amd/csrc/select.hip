@@ -342,4 +342,16 @@ hipError_t launch_to_user(const u32* ord, u64 m, int f32, void* out, hipStream_t
     return hipGetLastError();
 }
 
+unsigned device_cus() {
+    static unsigned cache[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cache[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cache[dev] = (unsigned)v;
+    }
+    return cache[dev];
+}
+
 }  // namespace drtopk
