@@ -127,6 +127,57 @@ def batched_leg(dt, torch, steps: int, warmup: int, nrows: int = 4096, graph_loo
     return res
 
 
+def served_leg(dt, torch, nreq: int = 2000, rate: float = 20000.0):
+    """Served workload: nreq requests, n log-uniform 2^10..2^26, coin-flip uint32 uniform /
+    float32 N(0,1), k log-uniform in [1, min(n/4, 2^16)], Poisson arrivals at `rate`, keys resident
+    in one device store, 64 GiB scratch arena.  The same trace is served (a) by the continuous-
+    batching server and (b) one request at a time by single-vector calls; both also saturated
+    (every request arriving at t = 0)."""
+    from drtopksc_amd.workloads import pack, serve_mix
+    mix = serve_mix(seed=8, nreq=nreq, rate_per_s=rate)
+    rows, keys_len, _ = pack([(n, k, d) for _, n, k, d in mix])
+    g = torch.Generator(device="cuda")
+    g.manual_seed(8)
+    store = torch.randint(-2 ** 31, 2 ** 31 - 1, (keys_len,), dtype=torch.int32, device="cuda", generator=g)
+    for off, n, k, oo, d in rows:
+        if d == 1:
+            store[off:off + n] = torch.randn(n, device="cuda", generator=g).view(torch.int32)
+    torch.cuda.synchronize()
+    qs = [(off, n, k, d) for off, n, k, oo, d in rows]
+    arr = [int(t * 1e9) for t, _, _, _ in mix]
+    sat = [0] * len(qs)
+    total = sum(n for _, n, _, _ in qs)
+
+    def summ(res, t0):
+        lat = np.array([(r.t_done_ns - r.t_submit_ns) / 1e3 for r in res])
+        span = (max(r.t_done_ns for r in res) - t0) / 1e9
+        assert all(r.status == 0 for r in res)
+        return {"req_s": round(len(res) / span, 1), "gelem_s": round(total / span / 1e9, 2),
+                "p50_us": round(float(np.percentile(lat, 50)), 1), "p99_us": round(float(np.percentile(lat, 99)), 1)}
+
+    with dt.Server(store) as srv:
+        srv.replay(qs[:64], [0] * 64)  # warm-up: code objects, pinned staging
+        r_rate, t0 = srv.replay(qs, arr)
+        served_rate = summ(r_rate, t0)
+        r_sat, t1 = srv.replay(qs, sat)
+        served_sat = summ(r_sat, t1)
+    s_rate, t2 = dt.replay_serial(store, qs, arr)
+    serial_rate = summ(s_rate, t2)
+    s_sat, t3 = dt.replay_serial(store, qs, sat)
+    serial_sat = summ(s_sat, t3)
+    identical = all(r_rate[i].kth_bits == s_rate[i].kth_bits == r_sat[i].kth_bits for i in range(len(qs)))
+    del store
+    torch.cuda.empty_cache()
+    return {"workload": f"{len(qs)} requests, n log-uniform 2^10..2^26, uint32 uniform / float32 N(0,1) coin flip, "
+                        f"k log-uniform [1, min(n/4, 2^16)], Poisson arrivals at {rate:.0f}/s, device-resident keys, "
+                        "64 GiB scratch arena",
+            "requests": len(qs), "keys": total, "rate_per_s": rate,
+            "served_at_rate": served_rate, "served_saturated": served_sat,
+            "serial_at_rate": serial_rate, "serial_saturated": serial_sat,
+            "speedup_vs_serial": round(served_sat["req_s"] / serial_sat["req_s"], 2),
+            "kth_identical": identical}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +188,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-rows", type=int, default=4096)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-requests", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU exchange is "
@@ -247,6 +300,13 @@ def main():
         torch.cuda.empty_cache()
         batched = batched_leg(dt, torch, args.steps, args.warmup, nrows=args.batch_rows)
         print(f"[bench] batched {batched}", file=sys.stderr, flush=True)
+    served = None
+    if not args.no_served:
+        if args.no_batched:
+            del dev, ws
+            torch.cuda.empty_cache()
+        served = served_leg(dt, torch, nreq=args.served_requests)
+        print(f"[bench] served {served}", file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Gelem/s", "n_gpus": 1, "steps": args.steps,
@@ -256,7 +316,7 @@ def main():
                                f"k=2^0..2^{args.kmax_log2}, keys+int64 indices out",
                    "n": n, "k_sweep": [ks[0], ks[-1]], "algo": "radix", "flags": "FILTER|BETA|SHUFFLE",
                    "digit_bits": 8, "beta": 2, "alpha": "auto (reference policy)"},
-        "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "served": served,
         "min_hbm_frac": min(r["hbm_frac"] for r in rows),
         "per_k": rows, "wall_s": round(wall, 3),
     }

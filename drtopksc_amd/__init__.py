@@ -8,5 +8,5 @@ if the library is missing or no GPU is present.
 from .api import (  # noqa: F401
     ALGO_BITONIC, ALGO_BUCKET, ALGO_RADIX, FLAG_BETA, FLAG_FILTER, FLAG_SHUFFLE, FLAG_TIMING, FLAGS_DEFAULT,
     TUNE_DEFAULT, U32, F32, Batch, DrTopKError, Params, Row, Stats, Tuning, lib, lib_path, params, sample_delegates,
-    select, tuning_default, tuning_from_table, workspace_size,
+    select, tuning_default, tuning_from_table, workspace_size, Server, Query, Result, replay_serial, device_to_host,
 )

@@ -254,3 +254,153 @@ class Batch:
                                            ctypes.c_void_p(kth.data_ptr()), ctypes.c_void_p(tk.data_ptr()),
                                            ctypes.c_void_p(ti.data_ptr()), _stream_ptr(stream)))
         return kth, tk, ti
+
+
+# ---------------------------------------------------------------- serving
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("d_store", ctypes.c_void_p), ("store_len", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
+                ("max_inflight", ctypes.c_uint32), ("max_batch_rows", ctypes.c_uint32), ("solo_n", ctypes.c_uint64),
+                ("solo_k", ctypes.c_uint64), ("tuning", ctypes.POINTER(Tuning)), ("reserved", ctypes.c_uint32 * 4)]
+
+
+class Query(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint64), ("n", ctypes.c_uint64), ("k", ctypes.c_uint64),
+                ("dtype", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("kth_bits", ctypes.c_uint32), ("d_topk_keys", ctypes.c_void_p),
+                ("d_topk_idx", ctypes.c_void_p), ("t_submit_ns", ctypes.c_uint64), ("t_launch_ns", ctypes.c_uint64),
+                ("t_done_ns", ctypes.c_uint64), ("launch_rows", ctypes.c_uint32), ("solo", ctypes.c_uint32)]
+
+
+_server_bound = False
+
+
+def _bind_server():
+    global _server_bound
+    if _server_bound:
+        return
+    L = lib()
+    vp, u64 = ctypes.c_void_p, ctypes.c_uint64
+    L.drtopk_server_config_default.argtypes = [ctypes.POINTER(ServerConfig)]
+    L.drtopk_server_config_default.restype = None
+    L.drtopk_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+    L.drtopk_server_create.restype = ctypes.c_int
+    L.drtopk_server_submit.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(u64)]
+    L.drtopk_server_submit.restype = ctypes.c_int
+    L.drtopk_server_wait.argtypes = [vp, u64, ctypes.c_int64, ctypes.POINTER(Result)]
+    L.drtopk_server_wait.restype = ctypes.c_int
+    L.drtopk_server_release.argtypes = [vp, u64]
+    L.drtopk_server_release.restype = ctypes.c_int
+    L.drtopk_server_destroy.argtypes = [vp]
+    L.drtopk_server_destroy.restype = ctypes.c_int
+    L.drtopk_server_replay.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(u64), u64, ctypes.POINTER(Result),
+                                       ctypes.POINTER(u64)]
+    L.drtopk_server_replay.restype = ctypes.c_int
+    L.drtopk_replay_serial.argtypes = [vp, u64, ctypes.POINTER(Query), ctypes.POINTER(u64), u64,
+                                       ctypes.POINTER(Result), ctypes.POINTER(u64)]
+    L.drtopk_replay_serial.restype = ctypes.c_int
+    _server_bound = True
+
+
+def _hip():
+    return ctypes.CDLL("libamdhip64.so")
+
+
+def device_to_host(ptr: int, nbytes: int, dtype):
+    """Copy nbytes at a device address into a new numpy array (hipMemcpy, device to host)."""
+    import numpy as np
+    out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+    if nbytes:
+        rc = _hip().hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(ptr), ctypes.c_size_t(nbytes), 2)
+        if rc != 0:
+            raise DrTopKError(4, f"hipMemcpy failed ({rc})")
+    return out
+
+
+def _queries(qs):
+    arr = (Query * len(qs))()
+    for i, (off, n, k, dt) in enumerate(qs):
+        arr[i] = Query(int(off), int(n), int(k), int(dt), 0)
+    return arr
+
+
+class Server:
+    """drtopk_server_*: continuous-batching top-k over a device key store (a 32-bit device tensor)."""
+
+    def __init__(self, store, budget_bytes=0, max_inflight=0, max_batch_rows=0, solo_n=0, solo_k=0, tuning=None):
+        _bind_server()
+        self.store = store
+        c = ServerConfig()
+        lib().drtopk_server_config_default(ctypes.byref(c))
+        c.d_store = store.data_ptr()
+        c.store_len = store.numel()
+        c.budget_bytes = budget_bytes
+        if max_inflight:
+            c.max_inflight = max_inflight
+        if max_batch_rows:
+            c.max_batch_rows = max_batch_rows
+        if solo_n:
+            c.solo_n = solo_n
+        if solo_k:
+            c.solo_k = solo_k
+        self._tuning = tuning
+        if tuning is not None:
+            c.tuning = ctypes.pointer(tuning)
+        h = ctypes.c_void_p()
+        _check(lib().drtopk_server_create(ctypes.byref(c), ctypes.byref(h)))
+        self.h = h
+
+    def submit(self, offset, n, k, dtype=U32) -> int:
+        q = Query(int(offset), int(n), int(k), int(dtype), 0)
+        t = ctypes.c_uint64()
+        _check(lib().drtopk_server_submit(self.h, ctypes.byref(q), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout_ns: int = -1) -> Result:
+        r = Result()
+        _check(lib().drtopk_server_wait(self.h, ticket, timeout_ns, ctypes.byref(r)))
+        return r
+
+    def fetch(self, r: Result, k: int):
+        """(kth bits, top-k keys as uint32 bits, int64 indices) of a waited, unreleased request."""
+        import numpy as np
+        keys = device_to_host(r.d_topk_keys, 4 * k, np.uint32)
+        idx = device_to_host(r.d_topk_idx, 8 * k, np.int64)
+        return r.kth_bits, keys, idx
+
+    def release(self, ticket: int):
+        _check(lib().drtopk_server_release(self.h, ticket))
+
+    def replay(self, queries, arrivals_ns):
+        """Replay [(offset, n, k, dtype)] at the given arrival offsets; returns (Result array, t0_ns)."""
+        qs = _queries(queries)
+        ar = (ctypes.c_uint64 * len(queries))(*[int(a) for a in arrivals_ns])
+        res = (Result * len(queries))()
+        t0 = ctypes.c_uint64()
+        _check(lib().drtopk_server_replay(self.h, qs, ar, len(queries), res, ctypes.byref(t0)))
+        return res, t0.value
+
+    def close(self):
+        if self.h:
+            _check(lib().drtopk_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def replay_serial(store, queries, arrivals_ns):
+    """One-at-a-time baseline over the same trace: (Result array, t0_ns)."""
+    _bind_server()
+    qs = _queries(queries)
+    ar = (ctypes.c_uint64 * len(queries))(*[int(a) for a in arrivals_ns])
+    res = (Result * len(queries))()
+    t0 = ctypes.c_uint64()
+    _check(lib().drtopk_replay_serial(ctypes.c_void_p(store.data_ptr()), store.numel(), qs, ar, len(queries), res,
+                                      ctypes.byref(t0)))
+    return res, t0.value

@@ -133,6 +133,13 @@ struct ListOf<2, 2> {
 // beta-1 delegates inside the subrange (8 bits each; u8 / u16 / u32 per subrange).
 template <int BETA, int POSM, class LT>
 __device__ __forceinline__ void store_delegates(const LT& t, u64 s, int alpha, u32* dvals, void* dpos) {
+    if constexpr (std::is_same<LT, Top2P>::value) {
+        // both delegates with one 8-byte store, the first maximum's offset as one byte
+        *(uint2*)(dvals + 2 * s) = make_uint2(t.m1, t.m2);
+        const u32 off = t.p1 == 0xFFFFFFFFu ? 0xFFu : (u32)(t.p1 - (u32)(s << alpha)) & 0xFFu;
+        ((uint8_t*)dpos)[s] = (uint8_t)off;
+        return;
+    }
     u32 packed = 0;
 #pragma unroll
     for (int j = 0; j < BETA; ++j) {
@@ -389,6 +396,58 @@ __device__ __forceinline__ void stage_fetch(const u32* __restrict__ keys, u64 n,
     }
 }
 
+// Subrange j of a full tile (every key valid), 2^A keys, A >= 2, read from LDS.
+template <int A, class LT>
+__device__ __forceinline__ void staged_reduce_full(const u32x4* tile, int j, u64 kb, LT& t) {
+    constexpr int QPS = 1 << (A - 2);
+    if constexpr (std::is_same<LT, Top2P>::value) {
+        u32 m1 = 0, m2 = 0, p1 = 0;
+#pragma unroll
+        for (int i = 0; i < QPS; ++i) {
+            const int q = j * QPS + i;
+            const u32x4 v = tile[q + (q >> 3)];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const u32 x = v[c];
+                if (i == 0 && c == 0) {
+                    m1 = x;
+                    continue;
+                }
+                m2 = max(m2, min(m1, x));
+                p1 = x > m1 ? (u32)(4 * i + c) : p1;
+                m1 = max(m1, x);
+            }
+        }
+        t.m1 = m1;
+        t.m2 = m2;
+        t.p1 = (u32)kb + p1;
+    } else {
+#pragma unroll
+        for (int i = 0; i < QPS; ++i) {
+            const int q = j * QPS + i;
+            const u32x4 v = tile[q + (q >> 3)];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) t.insert(v[c], kb + 4 * i + c);
+        }
+    }
+}
+
+template <int A, int BETA, int POSM, int FUSE, class LT>
+__device__ __forceinline__ void staged_tile_full(const u32x4* tile, u64 k0, u32* dvals, void* dpos, u32* lhist,
+                                                 int hshift, u32* mn, u32* mx) {
+    constexpr int SPT = STAGE_KEYS >> A;
+#pragma unroll
+    for (int j0 = 0; j0 < SPT; j0 += WG) {
+        const int j = j0 + threadIdx.x;
+        const u64 s = (k0 >> A) + j;
+        LT t;
+        t.clear();
+        staged_reduce_full<A, LT>(tile, j, k0 + ((u64)j << A), t);
+        store_delegates<BETA, POSM, LT>(t, s, A, dvals, dpos);
+        fuse_add<BETA, FUSE, LT>(t, true, lhist, hshift, mn, mx);
+    }
+}
+
 template <class K, int BETA, int POSM, int FUSE, class LT>
 __global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ keys, u64 n, int alpha, int vec_ok,
                                                       u64 ntiles, u32* __restrict__ dvals, void* __restrict__ dpos,
@@ -425,6 +484,16 @@ __global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ ke
         }
         __syncthreads();
         const u64 s0 = k0 >> alpha;
+        if (vec_ok && k0 + STAGE_KEYS <= n && alpha >= 2) {  // uniform: a full tile
+            switch (alpha) {
+                case 2: staged_tile_full<2, BETA, POSM, FUSE, LT>(tile, k0, dvals, dpos, lhist, hshift, &mn, &mx); break;
+                case 3: staged_tile_full<3, BETA, POSM, FUSE, LT>(tile, k0, dvals, dpos, lhist, hshift, &mn, &mx); break;
+                case 4: staged_tile_full<4, BETA, POSM, FUSE, LT>(tile, k0, dvals, dpos, lhist, hshift, &mn, &mx); break;
+                default: staged_tile_full<5, BETA, POSM, FUSE, LT>(tile, k0, dvals, dpos, lhist, hshift, &mn, &mx); break;
+            }
+            __syncthreads();
+            continue;
+        }
         for (int j = threadIdx.x; j < spt; j += WG) {  // uniform trip count: spt is a multiple of WG
             const u64 s = s0 + j;
             const u64 kb = k0 + ((u64)j << alpha);

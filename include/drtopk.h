@@ -201,6 +201,76 @@ drtopk_status drtopk_select_batched(const void* h_plan, const void* d_plan, cons
 /* Built-in tuning table (what tuning == NULL uses). */
 void drtopk_tuning_default(drtopk_tuning* t);
 
+/* ------------------------------------------------------------------------------------------
+ * Serving: a request loop over a device-resident key store with continuous batching.  The
+ * reference has no serving layer (its drivers run one vector per process, main.cu:56-323).
+ *
+ * A server owns a worker thread, max_inflight HIP streams and one device arena of
+ * budget_bytes that holds every launch's scratch and the requests' outputs.  Requests name a
+ * row of the key store (offset, n, k, dtype).  Whenever a stream slot is free the worker
+ * launches everything queued: requests with n >= solo_n or k > solo_k as single-vector calls
+ * (drtopk_select), the rest as one batched launch (drtopk_select_batched).  Requests that arrive
+ * while launches are running join the next launch as soon as a slot and arena space free up, so
+ * launches of different streams overlap on the device.  Admission: a request waits in the queue
+ * until its outputs and scratch fit the arena; one that could never fit is refused with
+ * DRTOPK_ERR_BUDGET.  Results are exactly drtopk_select's.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct drtopk_server drtopk_server;
+
+typedef struct drtopk_server_config {
+    const void* d_store;      /* device key store (16-byte aligned); requests index it */
+    uint64_t store_len;       /* its length in 32-bit elements */
+    uint64_t budget_bytes;    /* device arena for scratch and outputs (0: 64 GiB, capped at 90% of free memory) */
+    uint32_t max_inflight;    /* launches on the device at once, one stream and worker thread each (0: 8) */
+    uint32_t max_batch_rows;  /* rows per batched launch (0: 4096) */
+    uint64_t solo_n;          /* n at or above which a request runs as a single-vector call (0: 2^22) */
+    uint64_t solo_k;          /* k above which a request runs as a single-vector call (0: 4096) */
+    const drtopk_tuning* tuning; /* batched launches' tuning table (NULL: built-in) */
+    uint32_t reserved[4];     /* must be zero */
+} drtopk_server_config;
+
+typedef struct drtopk_query {
+    uint64_t offset;          /* first key in the store (elements, a multiple of 4) */
+    uint64_t n, k;
+    int32_t dtype;            /* drtopk_dtype */
+    int32_t reserved;         /* must be zero */
+} drtopk_query;
+
+typedef struct drtopk_result {
+    int32_t status;              /* drtopk_status of the request */
+    uint32_t kth_bits;           /* k-th largest key, 32-bit pattern of the request's dtype */
+    const void* d_topk_keys;     /* k keys (ascending index, lowest index wins ties), device, until release */
+    const int64_t* d_topk_idx;   /* their row-local indices, device, until release */
+    uint64_t t_submit_ns;        /* steady-clock times: submitted, launched, seen complete */
+    uint64_t t_launch_ns;
+    uint64_t t_done_ns;
+    uint32_t launch_rows;        /* requests in the launch that served it (single-vector call: 1) */
+    uint32_t solo;               /* 1: served by a single-vector call */
+} drtopk_result;
+
+void drtopk_server_config_default(drtopk_server_config* cfg);
+drtopk_status drtopk_server_create(const drtopk_server_config* cfg, drtopk_server** srv);
+/* Queue a request (thread-safe, does not block); *ticket identifies it. */
+drtopk_status drtopk_server_submit(drtopk_server* srv, const drtopk_query* q, uint64_t* ticket);
+/* Wait until the request is done (timeout_ns < 0: no limit); fills *res.  DRTOPK_ERR_BUDGET is
+ * returned when the wait timed out. */
+drtopk_status drtopk_server_wait(drtopk_server* srv, uint64_t ticket, int64_t timeout_ns, drtopk_result* res);
+/* Free the request's outputs (after wait). */
+drtopk_status drtopk_server_release(drtopk_server* srv, uint64_t ticket);
+/* Drain the queue, stop the worker, free the arena and streams. */
+drtopk_status drtopk_server_destroy(drtopk_server* srv);
+
+/* Load generator: submit q[i] at t0 + arrival_ns[i] (busy-wait pacing on the calling thread),
+ * then wait for and release every request; results[i] keeps the timestamps and kth.  *t0_ns
+ * receives t0. */
+drtopk_status drtopk_server_replay(drtopk_server* srv, const drtopk_query* q, const uint64_t* arrival_ns,
+                                   uint64_t count, drtopk_result* results, uint64_t* t0_ns);
+/* The one-at-a-time baseline: the same trace served first-come first-served by single-vector
+ * drtopk_select calls on one stream, each synchronised before the next starts. */
+drtopk_status drtopk_replay_serial(const void* d_store, uint64_t store_len, const drtopk_query* q,
+                                   const uint64_t* arrival_ns, uint64_t count, drtopk_result* results,
+                                   uint64_t* t0_ns);
+
 /* Thread-local message for the last error on this thread ("" if none). */
 const char* drtopk_last_error(void);
 

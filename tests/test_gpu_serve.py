@@ -1,0 +1,101 @@
+"""GPU tests of the served form (drtopk_server_*): every request's result in the mix equals the
+oracle's and drtopk_select's for the same request alone; admission by the arena budget."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def dt():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import drtopksc_amd as dt
+    dt.lib()
+    return dt
+
+
+def build_store(orc, specs, seed0=5000):
+    """specs [(n, k, dtype)] -> (device store, queries [(offset, n, k, dtype)], ordered host rows)."""
+    from drtopksc_amd.workloads import pack
+    rows, keys_len, _ = pack(specs)
+    buf = np.zeros(max(keys_len, 4), np.uint32)
+    ordered = []
+    for i, (off, n, k, oo, d) in enumerate(rows):
+        v = orc.generate(3 if d == 1 else 0, seed0 + i, n)
+        buf[off:off + n] = v
+        ordered.append(orc.f32_to_ordered(v) if d == 1 else v)
+    store = torch.from_numpy(buf.view(np.int32)).cuda()
+    return store, [(off, n, k, d) for off, n, k, oo, d in rows], ordered
+
+
+def check_one(orc, dt, srv, store, q, ordered, r, label):
+    off, n, k, d = q
+    kth, keys, idx = srv.fetch(r, k)
+    K, rkeys, ridx = orc.topk_reference(ordered, k)
+    kk = np.array([kth], np.uint32)
+    if d == 1:
+        kk = orc.f32_to_ordered(kk)
+        keys = orc.f32_to_ordered(keys)
+    assert int(kk[0]) == K, (label, q, "kth")
+    assert np.array_equal(idx, ridx), (label, q, "idx")
+    assert np.array_equal(keys, rkeys), (label, q, "keys")
+    # the same request alone through the single-vector entry point
+    x = store[off:off + n]
+    sk, skeys, sidx, _ = dt.select(x.view(torch.float32) if d == 1 else x, k)
+    assert int(sk.view(torch.int32).cpu().numpy().view(np.uint32)[0]) == kth, (label, q, "alone kth")
+    assert np.array_equal(sidx.cpu().numpy(), idx), (label, q, "alone idx")
+
+
+def test_mix_equals_alone(orc, dt):
+    from drtopksc_amd.workloads import serve_mix
+    specs = [(n, k, d) for _, n, k, d in serve_mix(seed=21, nreq=120, lo=10, hi=19, kmax=1 << 13)]
+    specs += [(1 << 20, 1 << 15, 0), (3 << 18, 100, 1), (1 << 19, 5000, 0)]
+    store, qs, ordered = build_store(orc, specs)
+    print(f"[case] serve mix requests={len(qs)}", flush=True)
+    with dt.Server(store, budget_bytes=1 << 30, solo_n=1 << 18, solo_k=4096) as srv:
+        tickets = [srv.submit(*q) for q in qs]
+        results = [srv.wait(t) for t in tickets]
+        rows = [r.launch_rows for r in results]
+        assert any(r.solo for r in results) and any(r.launch_rows > 1 for r in results), rows
+        for q, o, r, t in zip(qs, ordered, results, tickets):
+            check_one(orc, dt, srv, store, q, o, r, "mix")
+            assert r.t_submit_ns <= r.t_launch_ns <= r.t_done_ns
+            srv.release(t)
+
+
+def test_budget_admission(orc, dt):
+    specs = [(50000 + 997 * i, 1 + 31 * i, i % 2) for i in range(40)]
+    store, qs, ordered = build_store(orc, specs, seed0=6000)
+    print("[case] serve small budget", flush=True)
+    # 4 MiB holds one launch's scratch and outputs at a time: requests queue but all finish
+    with dt.Server(store, budget_bytes=4 << 20, max_batch_rows=8) as srv:
+        tickets = [srv.submit(*q) for q in qs]
+        for q, o, t in zip(qs, ordered, tickets):
+            r = srv.wait(t)
+            check_one(orc, dt, srv, store, q, o, r, "budget")
+            srv.release(t)
+        with pytest.raises(dt.DrTopKError) as e:
+            srv.submit(0, store.numel() - 4, store.numel() // 2, 0)
+        assert e.value.status == 6  # DRTOPK_ERR_BUDGET
+
+
+def test_replay_matches_serial(orc, dt):
+    from drtopksc_amd.workloads import serve_mix
+    mix = serve_mix(seed=22, nreq=80, lo=10, hi=20, kmax=1 << 12, rate_per_s=20000.0)
+    specs = [(n, k, d) for _, n, k, d in mix]
+    store, qs, ordered = build_store(orc, specs, seed0=7000)
+    arr = [int(t * 1e9) for t, _, _, _ in mix]
+    print(f"[case] serve replay requests={len(qs)}", flush=True)
+    with dt.Server(store, budget_bytes=1 << 30) as srv:
+        res, t0 = srv.replay(qs, arr)
+    ser, t1 = dt.replay_serial(store, qs, arr)
+    for i, (q, o) in enumerate(zip(qs, ordered)):
+        K, _, _ = orc.topk_reference(o, q[2])
+        a = np.array([res[i].kth_bits, ser[i].kth_bits], np.uint32)
+        if q[3] == 1:
+            a = orc.f32_to_ordered(a)
+        assert res[i].status == 0 and ser[i].status == 0
+        assert int(a[0]) == K and int(a[1]) == K, (i, q)
