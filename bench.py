@@ -178,6 +178,20 @@ def served_leg(dt, torch, nreq: int = 2000, rate: float = 20000.0):
             "kth_identical": identical}
 
 
+def side_legs_only(args):
+    """Run only the batched and/or served legs (for profiling); the headline value is null."""
+    import torch
+
+    import drtopksc_amd as dt
+    out = {"metric": METRIC, "value": None, "unit": "Gelem/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "note": "sweep leg skipped (--legs)"}
+    if not args.no_batched:
+        out["batched"] = batched_leg(dt, torch, args.steps, args.warmup, nrows=args.batch_rows)
+    if not args.no_served:
+        out["served"] = served_leg(dt, torch, nreq=args.served_requests)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,12 +203,19 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-rows", type=int, default=4096)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--legs", default="sweep,batched,served",
+                    help="comma list of sweep, batched, served (profiling runs pick one)")
     ap.add_argument("--served-requests", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU exchange is "
               "design-only in this build", file=sys.stderr)
         sys.exit(2)
+    legs = set(args.legs.split(","))
+    args.no_batched = args.no_batched or "batched" not in legs
+    args.no_served = args.no_served or "served" not in legs
+    if "sweep" not in legs:
+        return side_legs_only(args)
 
     import torch
 
@@ -275,8 +296,12 @@ def main():
     samp_bytes = 4 * n + 4 * st_ref["num_delegates"]
     samp_ms = samp[ref_i]
     achieved = samp_bytes / (samp_ms * 1e-3) / 1e9
+    # HBM bytes per launch of this kernel from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the
+    # same workload, FETCH_SIZE scaled by 2.000 and WRITE_SIZE by 1.000 (the factors a 4 GiB
+    # device-to-device copy reads back): profiles/delegate_pass_2p30_k1024.md
+    traffic = (2 * 2_097_187 + 4_398) * 1024 if args.log2n == 30 else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "delegate pass (k_sample_waves, alpha=11)", "bytes_per_launch": samp_bytes,
                 "avg_ms": round(samp_ms, 4)}
 

@@ -474,12 +474,22 @@ __global__ __launch_bounds__(BR_NT) void k_brows(const unsigned char* __restrict
 }
 
 // ---------------------------------------------------------------- tuning
+#include "tuning_table.inc"
+
+// The built-in table: cells measured by the autotuner on MI355X; every other cell keeps the
+// default choice below.
 void default_tuning(drtopk_tuning* t) {
     std::memset(t, DRTOPK_TUNE_DEFAULT, sizeof(*t));
+    for (const auto& c : kTunedCells) {
+        if (c[0] >= 32 || c[1] >= 32) continue;
+        t->alpha[c[0]][c[1]] = c[2];
+        t->digit_bits[c[0]][c[1]] = c[3];
+        t->tile_log2[c[0]][c[1]] = c[4];
+    }
 }
 
-// Built-in choice for a row: the reference's alpha policy (baseline+filter+beta+shuffle/
-// main.cu:83-89), 8-bit digits, 8192-key tiles.
+// Default choice for a cell the table leaves open: the reference's alpha policy
+// (baseline+filter+beta+shuffle/main.cu:83-89), 8-bit digits, 8192-key tiles.
 void choose(const drtopk_tuning& tu, u64 n, u64 k, int* alpha, int* nb, int* tile) {
     const int ln = ceil_log2(n), lk = ceil_log2(k);
     const u8 ta = tu.alpha[ln][lk], tn = tu.digit_bits[ln][lk], tt = tu.tile_log2[ln][lk];
