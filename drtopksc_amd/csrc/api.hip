@@ -252,6 +252,10 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
     const int D0 = 32 / nb - 1;
     const int bins = radix ? (1 << nb) : B;
     const bool fuse_hist = radix && nb <= 8;
+    // The first digit's histogram rides on the delegate pass, except for alpha 6..8: there the
+    // lane-group sampler produces one delegate per 16..64 lanes and the per-wave histogram update
+    // costs it more (25-60 us at 2^30 keys) than a separate pass over the delegates (11-17 us).
+    const bool fuse_first = fuse_hist && !(pl.alpha >= 6 && pl.alpha <= 8);
     const int Lmax = bucket_max_levels(B);
     const bool timing = (pl.flags & DRTOPK_FLAG_TIMING) && h_stats;
 
@@ -327,7 +331,7 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
         sa.keys = keys; sa.n = pl.n; sa.alpha = pl.alpha; sa.beta = pl.beta; sa.f32 = pl.f32;
         sa.staged = (pl.flags & DRTOPK_FLAG_SHUFFLE) && pl.alpha <= 5;
         sa.posmode = (pl.path == 2 && want_out) ? 2 : 0;
-        sa.fuse = radix ? (fuse_hist ? 1 : 0) : 2;
+        sa.fuse = radix ? (fuse_first ? 1 : 0) : 2;
         sa.dvals = dvals; sa.dpos = doff; sa.ghist = hist;
         sa.hshift = 32 - nb; sa.bins = bins; sa.ctrl = ctrl;
         HIPCHK(launch_sample(sa, st));
@@ -340,7 +344,7 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
             // (radix_select_inplace_firsttopk :1667-1724); beta path: digits >= 3 (:1747)
             const int ndig = pl.path == 1 ? D0 + 1 : std::max(0, D0 - 3 + 1);
             for (int d = 0; d < ndig; ++d) {
-                if (!(d == 0 && fuse_hist)) HIPCHK(launch_hist(0, dvals, fst, hist, bins, gd, st));
+                if (!(d == 0 && fuse_first)) HIPCHK(launch_hist(0, dvals, fst, hist, bins, gd, st));
                 HIPCHK(launch_select(0, hist, bins, fst, ctrl, st));
             }
         } else {

@@ -221,8 +221,9 @@ __device__ __forceinline__ void h2_flush(u32* lh, u32* ghist2, int bins2) {
 u64 chunks_per_segment(u64 nchunks) { return std::max<u64>(1, std::min<u64>(16, nchunks / 2048)); }
 
 // ---------------------------------------------------------------- default-path concatenation
+// alpha == 1: one subrange id per key pair; the plain loop below.
 template <class K, bool HIST2, bool IDX>
-__global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64 n, int alpha, int filter, int vec_ok,
+__global__ __launch_bounds__(WG) void k_gather_small(const u32* __restrict__ keys, u64 n, int alpha, int filter, int vec_ok,
                                                const u32* __restrict__ sel_ids, u64 nsel, u64 cps, Ctrl* ctrl,
                                                u32* __restrict__ skeys, u32* __restrict__ sidx, u32* __restrict__ cnt,
                                                u32* ghist2, int hshift2, int bins2) {
@@ -303,6 +304,117 @@ __global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64
     h2_flush<HIST2>(lh, ghist2, bins2);
 }
 
+
+// alpha >= 2: each thread owns TE = 16 consecutive slots of the virtual concatenation (the
+// selected subranges back to back), 4 granules of 4 keys that each lie in one subrange.  The
+// chunk loop is a register pipeline: subrange ids are loaded two chunks ahead and keys one chunk
+// ahead, so their latency overlaps the current chunk's scan, staging and copy-out (barriers on
+// gfx950 wait for LDS only, not for these loads).
+struct GatherIds {
+    u32 v[TE / 4];
+};
+struct GatherKeys {
+    u32x4 v[TE / 4];
+};
+
+__device__ __forceinline__ void gather_load_ids(const u32* __restrict__ sel_ids, u64 V, int alpha, u64 ch, GatherIds& g) {
+    const u64 v0 = ch * TT + (u64)threadIdx.x * TE;
+#pragma unroll
+    for (int q = 0; q < TE / 4; ++q) {
+        const u64 vp = v0 + 4 * q;
+        g.v[q] = vp < V ? sel_ids[vp >> alpha] : 0u;
+    }
+}
+
+__device__ __forceinline__ void gather_load_keys(const u32* __restrict__ keys, u64 n, u64 V, int alpha, int vec_ok,
+                                                 u64 ch, const GatherIds& g, GatherKeys& k) {
+    const u64 v0 = ch * TT + (u64)threadIdx.x * TE, Lm = (1ull << alpha) - 1;
+#pragma unroll
+    for (int q = 0; q < TE / 4; ++q) {
+        const u64 vp = v0 + 4 * q;
+        const u64 gq = ((u64)g.v[q] << alpha) + (vp & Lm);
+        u32x4 r = {0, 0, 0, 0};
+        if (vp < V) {
+            if (vec_ok && gq + 3 < n) {
+                r = *(const u32x4*)(keys + gq);
+            } else {
+                if (gq + 0 < n) r.x = keys[gq + 0];
+                if (gq + 1 < n) r.y = keys[gq + 1];
+                if (gq + 2 < n) r.z = keys[gq + 2];
+                if (gq + 3 < n) r.w = keys[gq + 3];
+            }
+        }
+        k.v[q] = r;
+    }
+}
+
+template <class K, bool HIST2, bool IDX>
+__global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64 n, int alpha, int filter, int vec_ok,
+                                               const u32* __restrict__ sel_ids, u64 nsel, u64 cps, Ctrl* ctrl,
+                                               u32* __restrict__ skeys, u32* __restrict__ sidx, u32* __restrict__ cnt,
+                                               u32* ghist2, int hshift2, int bins2) {
+    __shared__ u64 scratch[WG / WAVE + 1];
+    __shared__ u32 lh[HIST2 ? 256 : 1];
+    __shared__ u32 sk[STAGE];
+    __shared__ u32 si[IDX ? STAGE : 1];
+    h2_init<HIST2>(lh, bins2);
+    const u32 thr = ctrl->first.result;
+    const u64 Lm = (1ull << alpha) - 1;
+    const u64 V = nsel << alpha;
+    const u64 nchunks = (V + TT - 1) / TT;
+    const u64 nseg = (nchunks + cps - 1) / cps;
+    const u64 segcap = cps * TT;
+    for (u64 s = blockIdx.x; s < nseg; s += gridDim.x) {
+        u64 run = 0;
+        const u64 cb = s * cps, ce = std::min<u64>((s + 1) * cps, nchunks);
+        GatherIds ia, ib;
+        GatherKeys ka;
+        gather_load_ids(sel_ids, V, alpha, cb, ia);
+        if (cb + 1 < ce) gather_load_ids(sel_ids, V, alpha, cb + 1, ib);
+        gather_load_keys(keys, n, V, alpha, vec_ok, cb, ia, ka);
+        for (u64 ch = cb; ch < ce; ++ch) {
+            GatherIds ic;
+            GatherKeys kb;
+            if (ch + 2 < ce) gather_load_ids(sel_ids, V, alpha, ch + 2, ic);
+            if (ch + 1 < ce) gather_load_keys(keys, n, V, alpha, vec_ok, ch + 1, ib, kb);
+            const u64 v0 = ch * TT + (u64)threadIdx.x * TE;
+            u32 kv[TE];
+            u32 take = 0;  // bit j: element j passes
+#pragma unroll
+            for (int q = 0; q < TE / 4; ++q) {
+                const u64 vp = v0 + 4 * q;
+                const u64 gq = ((u64)ia.v[q] << alpha) + (vp & Lm);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const u32 o = K::to_ord(ka.v[q][c]);
+                    kv[4 * q + c] = o;
+                    const bool ok = vp + c < V && gq + c < n;
+                    if (ok && (!filter || o >= thr)) take |= 1u << (4 * q + c);
+                }
+            }
+            u64 tot;
+            u32 slot = (u32)block_excl_scan_u64<WG>((u64)__popc(take), scratch, &tot);
+#pragma unroll
+            for (int j = 0; j < TE; ++j) {
+                if (take & (1u << j)) {
+                    sk[slot] = kv[j];
+                    if (IDX) si[slot] = (u32)(((u64)ia.v[j >> 2] << alpha) + ((v0 + j) & Lm));
+                    ++slot;
+                }
+            }
+            __syncthreads();
+            flush_stage<IDX, HIST2>(sk, si, (u32)tot, skeys, sidx, s * segcap + run, lh, hshift2);
+            __syncthreads();
+            run += tot;
+            ia = ib;
+            ib = ic;
+            ka = kb;
+        }
+        if (threadIdx.x == 0) cnt[s] = (u32)run;
+    }
+    h2_flush<HIST2>(lh, ghist2, bins2);
+}
+
 hipError_t launch_gather(const GatherArgs& a, hipStream_t st) {
     const u64 V = a.nsel << a.alpha;
     const u64 nch = (V + TT - 1) / TT;
@@ -312,7 +424,11 @@ hipError_t launch_gather(const GatherArgs& a, hipStream_t st) {
     const int vec_ok = ((uintptr_t)a.keys & 15) == 0;
 #define G_ARGS a.keys, a.n, a.alpha, a.filter, vec_ok, a.sel_ids, a.nsel, cps, a.ctrl, a.skeys, a.sidx, a.cnt, \
                a.ghist2, a.hshift2, a.bins2
-#define G_LAUNCH(K, H, I) hipLaunchKernelGGL((k_gather<K, H, I>), dim3(grid), dim3(WG), 0, st, G_ARGS)
+#define G_LAUNCH(K, H, I)                                                                       \
+    do {                                                                                        \
+        if (a.alpha >= 2) hipLaunchKernelGGL((k_gather<K, H, I>), dim3(grid), dim3(WG), 0, st, G_ARGS); \
+        else hipLaunchKernelGGL((k_gather_small<K, H, I>), dim3(grid), dim3(WG), 0, st, G_ARGS);  \
+    } while (0)
     const bool h = a.ghist2 != nullptr, i = a.sidx != nullptr;
     if (a.f32) {
         if (h && i) G_LAUNCH(KeyF32, true, true); else if (h) G_LAUNCH(KeyF32, true, false);
