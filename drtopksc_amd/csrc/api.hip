@@ -367,7 +367,10 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
                           ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins};
             HIPCHK(launch_gather(ga, st));
         } else {
-            BetaGatherArgs ba{keys, pl.n, pl.alpha, pl.beta, pl.f32, (pl.flags & DRTOPK_FLAG_FILTER) ? 1 : 0,
+            // filter bit 1: keep keys >= threshold; bit 2 (radix family): lonely delegates below the
+            // threshold are counted, not written
+            const int bfilter = (pl.flags & DRTOPK_FLAG_FILTER) ? (radix ? 3 : 1) : 0;
+            BetaGatherArgs ba{keys, pl.n, pl.alpha, pl.beta, pl.f32, bfilter,
                               dvals, doff, ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins};
             HIPCHK(launch_beta_gather(ba, st));
         }

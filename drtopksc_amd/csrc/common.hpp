@@ -69,7 +69,8 @@ struct Ctrl {
     SelState second;      // second top-k over the concatenation (or over V)
     u64 selected;         // selected subranges
     u64 lonely_sub;       // lonely subranges (beta path)
-    u64 lonely_el;        // lonely delegates appended
+    u64 lonely_el;        // lonely delegates appended (as the reference counts them)
+    u64 lonely_dropped;   // of those, below the threshold and not written (FILTER)
     u64 concat_size;      // keys in the concatenation
     u64 count_gt;         // keys > k-th largest
     u64 ties;             // keys == k-th largest in the top-k set

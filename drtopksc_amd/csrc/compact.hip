@@ -524,7 +524,7 @@ __global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys
                                                     int hshift2, int bins2) {
     __shared__ u64 scratch[WG / WAVE + 1];
     __shared__ u32 lh[HIST2 ? 256 : 1];
-    __shared__ u64 red[3][WG / WAVE];
+    __shared__ u64 red[4][WG / WAVE];
     __shared__ u32 sk[BSTAGE];
     __shared__ u32 si[POS ? BSTAGE : 1];
     __shared__ uint16_t slist[BETA_TILE_SUBS];  // selected subranges of the chunk (chunk-local ids), ascending
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const int G = alpha >= 2 ? (1 << (alpha - 2)) : 1;  // lanes per listed subrange (alpha <= 8)
     const int gl = lane & (G - 1), grp = lane / G, per_wave = WAVE / G;
-    u64 nsel = 0, nlon = 0, nlel = 0;
+    u64 nsel = 0, nlon = 0, nlel = 0, ndrop = 0;
     for (u64 sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
         u64 run = 0;
         const u64 ch_end = std::min<u64>((sg + 1) * cps, nchunks);
@@ -557,7 +557,7 @@ __global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys
                 if (POS) load_doff<BETA>(doff, s0 + BETA_TILE_SUBS, S, pkn);
             }
             int kind[BSPT];  // 0 none, 1 selected (its keys), 2 lonely (its delegates)
-            u32 mj[BSPT];    // delegates of the subrange (min(beta, its length))
+            u32 mj[BSPT];    // lonely: delegates written (with FILTER only those >= threshold)
             u32 ns_t = 0;
 #pragma unroll
             for (int j = 0; j < BSPT; ++j) {
@@ -579,6 +579,17 @@ __global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys
                     kind[j] = 2;
                     ++nlon;
                     nlel += (u64)m;
+                    if (filter & 2) {
+                        // a lonely delegate below the threshold is below the k-th largest key:
+                        // counted in the concatenation's size, not written (radix family; the
+                        // bucket family's second pass maps the concatenation's min..max range, so
+                        // there every delegate is written as the reference does)
+                        u32 ge = 0;
+#pragma unroll
+                        for (int b = 0; b < BETA; ++b) ge += (b < m && d[j][b] >= thr) ? 1u : 0u;
+                        mj[j] = ge;
+                        ndrop += (u64)(m - (int)ge);
+                    }
                 }
             }
             // (1) list the selected subranges
@@ -727,7 +738,7 @@ __global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys
                                 }
 #pragma unroll
                         for (int b = 0; b < BETA; ++b)
-                            if (b < m) put(slot++, dv[b], p[b]);
+                            if (b < (int)mj[j]) put(slot++, dv[b], p[b]);
                     }
                 }
             }
@@ -784,15 +795,17 @@ __global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys
             nsel += shfl_xor_u64(nsel, o);
             nlon += shfl_xor_u64(nlon, o);
             nlel += shfl_xor_u64(nlel, o);
+            ndrop += shfl_xor_u64(ndrop, o);
         }
-        if (lane == 0) { red[0][w] = nsel; red[1][w] = nlon; red[2][w] = nlel; }
+        if (lane == 0) { red[0][w] = nsel; red[1][w] = nlon; red[2][w] = nlel; red[3][w] = ndrop; }
         __syncthreads();
         if (threadIdx.x == 0) {
-            u64 a = 0, b = 0, cc = 0;
-            for (int i = 0; i < WG / WAVE; ++i) { a += red[0][i]; b += red[1][i]; cc += red[2][i]; }
+            u64 a = 0, b = 0, cc = 0, dd = 0;
+            for (int i = 0; i < WG / WAVE; ++i) { a += red[0][i]; b += red[1][i]; cc += red[2][i]; dd += red[3][i]; }
             if (a) atomicAdd((unsigned long long*)&ctrl->selected, (unsigned long long)a);
             if (b) atomicAdd((unsigned long long*)&ctrl->lonely_sub, (unsigned long long)b);
             if (cc) atomicAdd((unsigned long long*)&ctrl->lonely_el, (unsigned long long)cc);
+            if (dd) atomicAdd((unsigned long long*)&ctrl->lonely_dropped, (unsigned long long)dd);
         }
     }
     h2_flush<HIST2>(lh, ghist2, bins2);
@@ -846,7 +859,7 @@ __global__ __launch_bounds__(SCAN_T) void k_seg_total(const u32* __restrict__ cn
     u64 tot;
     block_excl_scan_u64<SCAN_T>(s, scratch, &tot);
     if (threadIdx.x == 0) {
-        ctrl->concat_size = tot;
+        ctrl->concat_size = tot + ctrl->lonely_dropped;  // the dropped lonely delegates count too
         ctrl->second.m = tot;
         if (tot < ctrl->second.k) {
             atomicOr(&ctrl->error, ERR_CONCAT_SHORT);

@@ -95,7 +95,7 @@ u64 gather_nseg(u64 V);
 struct BetaGatherArgs {
     const u32* keys;
     u64 n;
-    int alpha, beta, f32, filter;
+    int alpha, beta, f32, filter;  // filter: bit 0 keep keys >= threshold, bit 1 drop lonely delegates below it
     const u32* dvals;
     const void* doff;
     Ctrl* ctrl;
