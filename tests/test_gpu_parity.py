@@ -202,3 +202,26 @@ def test_pipeline_bitonic(orc, dt, name):
     for k, alpha in [(1, -1), (2, 10), (16, -1), (128, 12), (1024, 8), (4096, 5), (8192, 6), (1 << 14, 5),
                      (64, 0), (8192, 0)]:
         compare(orc, dt, name, ok, dev, k, 2, orc.FLAGS_DEFAULT, alpha=alpha, f32=f32)
+
+
+@pytest.mark.parametrize("dist,seed", [(3, 5), (4, 6)])
+def test_f32_2p28_bitonic_and_radix(orc, dt, dist, seed):
+    """|V| = 2^28 float32 keys, N(0,1) and U(-1,1): bitonic top-k (alpha from 16 down to 5 as k
+    grows) and radix select on the sign-flipped bits give the oracle's exact top-k."""
+    n = 1 << 28
+    v = orc.generate(dist, seed, n)
+    ok = orc.f32_to_ordered(v)
+    dev = to_dev_f32(v)
+    for k in (1 << 4, 1 << 10, 1 << 16, 1 << 20):
+        _, K, keys, idx = orc.cpu_baseline(ok, k, threads=0)
+        for algo in (2, 0):
+            print(f"[case] f32 2^28 dist={dist} k={k} algo={algo}", flush=True)
+            kth, tk, ti, st = dt.select(dev, k, algo=algo, stats=True)
+            torch.cuda.synchronize()
+            assert int(orc.f32_to_ordered(kth.cpu().numpy().view(np.uint32))[0]) == K, (k, algo)
+            assert np.array_equal(ti.cpu().numpy(), idx), (k, algo, "idx")
+            assert np.array_equal(orc.f32_to_ordered(tk.cpu().numpy().view(np.uint32)), keys), (k, algo, "keys")
+            if algo == 2:
+                assert 5 <= st["alpha"] <= 16
+    del dev
+    torch.cuda.empty_cache()
