@@ -449,10 +449,10 @@ u64 gather_nseg(u64 V) {
 }
 
 // ---------------------------------------------------------------- beta-path range selection + concatenation
-constexpr int BSPT = 8;  // subranges per thread (2048 per chunk)
+constexpr int BSPT = DRTOPK_BG_BSPT;  // subranges per thread of the beta gather
 
-// Per chunk of 2048 subranges: (1) each thread classifies its 8 subranges from their
-// delegates and packed offsets (prefetched one chunk ahead) and the selected ones are listed in LDS in id order;
+// Per chunk of 1024 subranges: (1) each thread classifies its 4 subranges from their
+// delegates and packed offsets and the selected ones are listed in LDS in id order;
 // (2) the listed subranges' keys are copied to LDS with 16-byte loads (re-read from global
 // memory instead when they exceed the buffer) and their kept keys counted, 2^alpha/4 lanes
 // per subrange; (3) a block scan places every subrange's output (selected: its kept keys,
@@ -484,23 +484,34 @@ __device__ __forceinline__ void load_deleg(const u32* __restrict__ dvals, u64 s0
 }
 
 // Packed in-subrange offsets of the first beta-1 delegates of the thread's BSPT subranges
-// (u8 / u16 / u32 per subrange for beta 2 / 3 / 4).
+// (u8 / u16 / u32 per subrange for beta 2 / 3 / 4): BSPT * {1,2,4} bytes at an offset that is a
+// multiple of that size, read with one aligned load.
 template <int BETA>
 __device__ __forceinline__ void load_doff(const void* __restrict__ doff, u64 s0, u64 S, u32 (&pk)[BSPT]) {
+    static_assert(BSPT == 4 || BSPT == 8, "packed-offset loads assume 4 or 8 subranges per thread");
+    constexpr int EB = BETA == 2 ? 1 : (BETA == 3 ? 2 : 4);  // bytes per subrange
     if (s0 + BSPT <= S) {
-        if (BETA == 2) {
-            const u64 q = *(const u64*)((const uint8_t*)doff + s0);
+        const unsigned char* p = (const unsigned char*)doff + s0 * EB;
+        if constexpr (EB * BSPT == 4) {
+            const u32 q = *(const u32*)p;
 #pragma unroll
-            for (int j = 0; j < BSPT; ++j) pk[j] = (u32)(q >> (8 * j)) & 0xFFu;
-        } else if (BETA == 3) {
-            const u32x4 q = *(const u32x4*)((const uint16_t*)doff + s0);
+            for (int j = 0; j < BSPT; ++j) pk[j] = (q >> (8 * j)) & 0xFFu;
+        } else if constexpr (EB * BSPT == 8) {
+            const u64 q = *(const u64*)p;
 #pragma unroll
-            for (int j = 0; j < BSPT; ++j) pk[j] = (q[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+            for (int j = 0; j < BSPT; ++j) pk[j] = (u32)(q >> (8 * EB * j)) & (EB == 1 ? 0xFFu : 0xFFFFu);
         } else {
-            const u32x4 q0 = *(const u32x4*)((const u32*)doff + s0);
-            const u32x4 q1 = *(const u32x4*)((const u32*)doff + s0 + 4);
+            u32 w[EB * BSPT / 4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) { pk[j] = q0[j]; pk[4 + j] = q1[j]; }
+            for (int q = 0; q < EB * BSPT / 16; ++q) {
+                const u32x4 r = *(const u32x4*)(p + 16 * q);
+                w[4 * q] = r.x; w[4 * q + 1] = r.y; w[4 * q + 2] = r.z; w[4 * q + 3] = r.w;
+            }
+#pragma unroll
+            for (int j = 0; j < BSPT; ++j) {
+                if (EB == 4) pk[j] = w[j];
+                else pk[j] = (w[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+            }
         }
     } else {
 #pragma unroll
@@ -544,18 +555,12 @@ __global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys
     for (u64 sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
         u64 run = 0;
         const u64 ch_end = std::min<u64>((sg + 1) * cps, nchunks);
-        u32 d[BSPT][BETA], pk[BSPT];
-        load_deleg<BETA>(dvals, sg * cps * BETA_TILE_SUBS + (u64)threadIdx.x * BSPT, S, d);
-        if (POS) load_doff<BETA>(doff, sg * cps * BETA_TILE_SUBS + (u64)threadIdx.x * BSPT, S, pk);
         for (u64 ch = sg * cps; ch < ch_end; ++ch) {
             const u64 c0 = ch * BETA_TILE_SUBS;
             const u64 s0 = c0 + (u64)threadIdx.x * BSPT;
-            u32 dn[BSPT][BETA], pkn[BSPT];
-            const bool more = ch + 1 < ch_end;
-            if (more) {
-                load_deleg<BETA>(dvals, s0 + BETA_TILE_SUBS, S, dn);
-                if (POS) load_doff<BETA>(doff, s0 + BETA_TILE_SUBS, S, pkn);
-            }
+            u32 d[BSPT][BETA], pk[BSPT];
+            load_deleg<BETA>(dvals, s0, S, d);
+            if (POS) load_doff<BETA>(doff, s0, S, pk);
             int kind[BSPT];  // 0 none, 1 selected (its keys), 2 lonely (its delegates)
             u32 mj[BSPT];    // lonely: delegates written (with FILTER only those >= threshold)
             u32 ns_t = 0;
@@ -777,14 +782,6 @@ __global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys
             if (staged) flush_stage<POS, HIST2>(sk, si, (u32)tot, skeys, sidx, gbase, lh, hshift2);
             __syncthreads();
             run += tot;
-            if (more) {
-#pragma unroll
-                for (int j = 0; j < BSPT; ++j) {
-#pragma unroll
-                    for (int b = 0; b < BETA; ++b) d[j][b] = dn[j][b];
-                    pk[j] = pkn[j];
-                }
-            }
         }
         if (threadIdx.x == 0) cnt[sg] = (u32)run;
     }

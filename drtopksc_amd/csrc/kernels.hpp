@@ -105,7 +105,10 @@ struct BetaGatherArgs {
     u32* ghist2;
     int hshift2, bins2;
 };
-constexpr u64 BETA_TILE_SUBS = 2048;
+#ifndef DRTOPK_BG_BSPT
+#define DRTOPK_BG_BSPT 4  // 4 measured faster than 8 (fewer VGPRs, more resident workgroups)
+#endif
+constexpr u64 BETA_TILE_SUBS = 256 * DRTOPK_BG_BSPT;  // subranges per chunk of the beta gather
 hipError_t launch_beta_gather(const BetaGatherArgs& a, hipStream_t st);
 u64 beta_segcap(u64 S, int alpha);
 u64 beta_nseg(u64 S);
