@@ -156,24 +156,41 @@ hipError_t launch_hist_seg(int mode, const Seg& seg, SelState* sel, u32* ghist, 
 
 // ---------------------------------------------------------------- select step
 // Target: the first bin i whose cumulative count (within this pass) reaches T = m-k+1-below,
-// i.e. the bin holding the key of ascending rank m-k.
-constexpr int SEL_T = 1024;
+// i.e. the bin holding the key of ascending rank m-k.  One 256-thread workgroup: with <= 256
+// bins each thread owns one bin and issues all of its HIST_REPS replica loads at once; larger
+// (bucket) histograms have one replica and a few bins per thread.
+constexpr int SEL_T = 256;
 
 template <int MODE>
 __global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState* sel, Ctrl* ctrl) {
     __shared__ u64 scratch[SEL_T / WAVE + 1];
     __shared__ u64 found[3];  // bin, count before it, count in it
     if (sel->done) return;
+    constexpr int MAXPER = MAX_LDS_BINS / SEL_T;  // 16
     const int per = (bins + SEL_T - 1) / SEL_T;
     const int b0 = min(bins, (int)threadIdx.x * per), b1 = min(bins, b0 + per);
-    const int reps = hist_reps(bins);
-    auto bin_count = [&](int b) -> u64 {
-        u64 c = 0;
-        for (int r = 0; r < reps; ++r) c += ghist[(size_t)r * bins + b];
-        return c;
-    };
+    u32 cnt[MAXPER];
     u64 sum = 0;
-    for (int b = b0; b < b1; ++b) sum += bin_count(b);
+    if (bins <= SEL_T) {  // 16 replicas, one bin per thread
+        u32 c = 0;
+        if (b0 < b1) {
+            u32 r16[HIST_REPS];
+#pragma unroll
+            for (int r = 0; r < HIST_REPS; ++r) r16[r] = ghist[(size_t)r * bins + b0];
+#pragma unroll
+            for (int r = 0; r < HIST_REPS; ++r) c += r16[r];
+        }
+        cnt[0] = c;
+        sum = c;
+    } else if (per <= MAXPER) {  // one replica, up to 16 bins per thread
+#pragma unroll
+        for (int j = 0; j < MAXPER; ++j) {
+            cnt[j] = (b0 + j < b1) ? ghist[b0 + j] : 0u;
+            sum += cnt[j];
+        }
+    } else {  // 16-bit digits: one replica, 256 bins per thread, read again below
+        for (int b = b0; b < b1; ++b) sum += ghist[b];
+    }
     u64 tot;
     const u64 excl = block_excl_scan_u64<SEL_T>(sum, scratch, &tot);
     const u64 m = sel->m, k = sel->k, below = sel->below;
@@ -182,13 +199,34 @@ __global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState
     __syncthreads();
     if (excl < T && excl + sum >= T) {
         u64 run = excl;
-        for (int b = b0; b < b1; ++b) {
-            u64 c = bin_count(b);
-            if (run + c >= T) { found[0] = (u64)b; found[1] = run; found[2] = c; break; }
-            run += c;
+        if (per <= MAXPER) {
+#pragma unroll
+            for (int j = 0; j < MAXPER; ++j) {
+                if (b0 + j >= b1) break;
+                const u64 c = cnt[j];
+                if (run + c >= T) {
+                    found[0] = (u64)(b0 + j);
+                    found[1] = run;
+                    found[2] = c;
+                    break;
+                }
+                run += c;
+            }
+        } else {
+            for (int b = b0; b < b1; ++b) {
+                const u64 c = ghist[b];
+                if (run + c >= T) {
+                    found[0] = (u64)b;
+                    found[1] = run;
+                    found[2] = c;
+                    break;
+                }
+                run += c;
+            }
         }
     }
     __syncthreads();
+    const int reps = hist_reps(bins);
     for (int r = 0; r < reps; ++r)  // ready for the next pass
         for (int b = b0; b < b1; ++b) ghist[(size_t)r * bins + b] = 0;
     if (threadIdx.x != 0) return;
