@@ -458,8 +458,11 @@ constexpr int BSPT = DRTOPK_BG_BSPT;  // subranges per thread of the beta gather
 // per subrange; (3) a block scan places every subrange's output (selected: its kept keys,
 // lonely: its delegates, in index order); (4) each listed key goes to its slot by a prefix
 // sum over its subrange's lanes.  Survivors are staged in LDS and leave with coalesced stores.
-constexpr u32 BSTAGE = 2048;  // staged survivors per chunk (more: direct stores)
-constexpr u32 KBUF = 2048;    // keys of listed subranges held in LDS
+#ifndef DRTOPK_BG_STAGE
+#define DRTOPK_BG_STAGE 1024  // with 5 waves per SIMD below: 2-4% faster than 2048 at 4 waves
+#endif
+constexpr u32 BSTAGE = DRTOPK_BG_STAGE;  // staged survivors per chunk (more: direct stores)
+constexpr u32 KBUF = DRTOPK_BG_STAGE;  // keys of listed subranges held in LDS
 
 template <int BETA>
 __device__ __forceinline__ void load_deleg(const u32* __restrict__ dvals, u64 s0, u64 S, u32 (&d)[BSPT][BETA]) {
@@ -527,8 +530,11 @@ __device__ __forceinline__ void load_doff(const void* __restrict__ doff, u64 s0,
     }
 }
 
+#ifndef DRTOPK_BG_WAVES
+#define DRTOPK_BG_WAVES 5  // fits 96 VGPRs without spilling; 6 and 8 spill and run slower
+#endif
 template <class K, int BETA, bool POS, bool HIST2>
-__global__ __launch_bounds__(WG) void k_beta_gather(const u32* __restrict__ keys, u64 n, int alpha, int filter,
+__global__ __launch_bounds__(WG, DRTOPK_BG_WAVES) void k_beta_gather(const u32* __restrict__ keys, u64 n, int alpha, int filter,
                                                     const u32* __restrict__ dvals, const void* __restrict__ doff,
                                                     u64 cps, Ctrl* ctrl, u32* __restrict__ skeys,
                                                     u32* __restrict__ sidx, u32* __restrict__ cnt, u32* ghist2,

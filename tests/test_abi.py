@@ -101,3 +101,87 @@ def test_bitonic_validation(L):
     assert "power-of-two" in L.drtopk_last_error().decode()
     assert ws(L, 2, 1 << 20, 1 << 16, alpha=8)[0] == 1  # explicit alpha 8 leaves 4096 subranges < k
     assert ws(L, 2, 1 << 20, 1 << 16)[0] == 0           # auto alpha is reduced until S > k
+
+
+def _rows(api, specs):
+    arr = (api.Row * len(specs))()
+    for i, (off, n, k, oo, d) in enumerate(specs):
+        arr[i] = api.Row(off, n, k, oo, d, 0)
+    return arr
+
+
+def test_batch_plan_validation(L):
+    """drtopk_batch_plan is host-only: layout and validation without a GPU."""
+    from drtopksc_amd import api
+    pb, wb = ctypes.c_size_t(), ctypes.c_size_t()
+
+    def plan(specs, keys_len=1 << 20, out_len=1 << 20, tuning=None):
+        rows = _rows(api, specs)
+        return L.drtopk_batch_plan(rows, len(specs), keys_len, out_len,
+                                   ctypes.byref(tuning) if tuning is not None else None, None, ctypes.byref(pb),
+                                   ctypes.byref(wb))
+
+    assert plan([(0, 5000, 10, 0, 0), (5000, 7000, 100, 10, 1)]) == 0 and pb.value > 128 and wb.value >= 256
+    assert plan([(2, 5000, 10, 0, 0)]) == 1 and "multiple of 4" in L.drtopk_last_error().decode()
+    assert plan([(0, 5000, 0, 0, 0)]) == 1           # k == 0
+    assert plan([(0, 5000, 5001, 0, 0)]) == 1        # k > n
+    assert plan([(0, 5000, 10, 0, 7)]) == 1          # bad dtype
+    assert plan([(0, 5000, 10, 0, 0)], keys_len=4000) == 1   # row outside the key buffer
+    assert plan([(0, 5000, 10, 0, 0)], out_len=5) == 1       # outputs outside the output arrays
+    t = api.tuning_default()
+    t.digit_bits[13][4] = 5
+    assert plan([(0, 5000, 10, 0, 0)], tuning=t) == 1 and "digit_bits" in L.drtopk_last_error().decode()
+    t = api.tuning_default()
+    t.tile_log2[13][4] = 9
+    assert plan([(0, 5000, 10, 0, 0)], tuning=t) == 1
+
+
+def test_batch_plan_layout(L):
+    """The plan blob: header, one 64-byte record per row, the tile -> row map, the row order."""
+    import struct
+
+    import numpy as np
+
+    from drtopksc_amd import api
+    specs = [(0, 70000, 100, 0, 0), (70000, 4, 4, 100, 1), (70004, 1 << 18, 4096, 104, 0)]
+    rows = _rows(api, specs)
+    pb, wb = ctypes.c_size_t(), ctypes.c_size_t()
+    assert L.drtopk_batch_plan(rows, 3, 1 << 20, 4200, None, None, ctypes.byref(pb), ctypes.byref(wb)) == 0
+    buf = np.zeros(pb.value, np.uint8)
+    assert L.drtopk_batch_plan(rows, 3, 1 << 20, 4200, None, ctypes.c_void_p(buf.ctypes.data), ctypes.byref(pb),
+                               ctypes.byref(wb)) == 0
+    hdr = struct.unpack_from("<12Q", buf.tobytes(), 0)
+    magic, nrows, nchunks, o_rows, o_crow, o_order = hdr[:6]
+    assert nrows == 3 and o_rows == 128
+    recs = [struct.unpack_from("<6Q3I4B", buf.tobytes(), o_rows + 64 * i) for i in range(3)]
+    tiles = 0
+    for (key_off, n, k, out_off, deleg_off, sel_off, S, cb, nc, alpha, f32, nb, tl), (off, nn, kk, oo, d) in zip(
+            recs, specs):
+        assert (key_off, n, k, out_off, f32) == (off, nn, kk, oo, d)
+        if alpha:
+            assert S == -(-n // (1 << alpha)) and S >= k and alpha <= tl
+            assert cb == tiles and nc == -(-n // (1 << tl))
+            tiles += nc
+        assert nb in (4, 8)
+    assert nchunks == tiles
+    crow = np.frombuffer(buf.tobytes(), np.uint32, nchunks, o_crow)
+    for r, rec in enumerate(recs):
+        assert np.all(crow[rec[7]:rec[7] + rec[8]] == r)
+    order = np.frombuffer(buf.tobytes(), np.uint32, 3, o_order)
+    assert sorted(order.tolist()) == [0, 1, 2] and order[0] == 2  # largest row first
+
+
+def test_server_validation(L):
+    """drtopk_server_create refuses bad configurations before touching the GPU."""
+    from drtopksc_amd import api
+    api._bind_server()
+    h = ctypes.c_void_p()
+    c = api.ServerConfig()
+    L.drtopk_server_config_default(ctypes.byref(c))
+    assert (c.max_inflight, c.max_batch_rows, c.solo_n, c.solo_k) == (8, 4096, 1 << 22, 4096)
+    assert L.drtopk_server_create(ctypes.byref(c), ctypes.byref(h)) == 1  # no key store
+    c.d_store = 4
+    assert L.drtopk_server_create(ctypes.byref(c), ctypes.byref(h)) == 1  # misaligned
+    c.d_store = 256
+    c.reserved[0] = 1
+    assert L.drtopk_server_create(ctypes.byref(c), ctypes.byref(h)) == 1
