@@ -924,7 +924,10 @@ __global__ __launch_bounds__(WG) void k_extract_write(Seg g, SelState* sel, u64 
         for (u64 c0 = 0; c0 < len; c0 += TT) {
             const u64 e0 = c0 + (u64)threadIdx.x * TE;
             u32 v[TE];
+            u32 vi[(IDX && !DIRECT) ? TE : 1];
             load_te<K, DIRECT>(base + e0, e0, len, v);
+            // the concatenation's indices, read with the keys (16-byte loads) rather than one by one later
+            if (IDX && !DIRECT) load_te<KeyU32, false>(g.idx + s * g.segcap + e0, e0, len, vi);
             u32 gt = 0, eq = 0;
 #pragma unroll
             for (int j = 0; j < TE; ++j) {
@@ -952,7 +955,7 @@ __global__ __launch_bounds__(WG) void k_extract_write(Seg g, SelState* sel, u64 
                 if (pos != ~0ull) {
                     const u32 sl = (u32)(pos - P0);
                     if (KEYS) sk[sl] = K::from_ord(v[j]);
-                    if (IDX) si[sl] = DIRECT ? (u32)(s * g.segcap + e0 + j) : g.idx[s * g.segcap + e0 + j];
+                    if (IDX) si[sl] = DIRECT ? (u32)(s * g.segcap + e0 + j) : vi[(IDX && !DIRECT) ? j : 0];
                 }
             }
             __syncthreads();
