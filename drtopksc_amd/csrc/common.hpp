@@ -115,13 +115,16 @@ __device__ __forceinline__ u64 wave_incl_scan_u64(u64 v) {
     }
     return v;
 }
+// 32-bit inclusive prefix sum over the 64 lanes of a wave with DPP: row_shr 1/2/4/8 scans each
+// row of 16 lanes, row_bcast:15 adds row 0's (row 2's) total to row 1 (row 3), row_bcast:31
+// adds lane 31's running total to rows 2 and 3.  Source lanes outside the row read the 0 `old`.
 __device__ __forceinline__ u32 wave_incl_scan_u32(u32 v) {
-    const int lane = threadIdx.x & (WAVE - 1);
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-        u32 t = (u32)__shfl_up((int)v, o, WAVE);
-        if (lane >= o) v += t;
-    }
+    v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
     return v;
 }
 
@@ -153,13 +156,36 @@ __device__ __forceinline__ u64 block_excl_scan_u64(u64 v, u64* scratch, u64* tot
     return off + inc - v;
 }
 
+// u32 form of block_excl_scan_u64 (the block sum must fit in 32 bits); `scratch` needs NT/64
+// u32 slots of LDS.  Contains __syncthreads.
+template <int NT>
+__device__ __forceinline__ u32 block_excl_scan_u32(u32 v, u32* scratch, u32* total) {
+    const int w = threadIdx.x / WAVE;
+    const u32 inc = wave_incl_scan_u32(v);
+    if ((threadIdx.x & (WAVE - 1)) == WAVE - 1) scratch[w] = inc;
+    __syncthreads();
+    u32 off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / WAVE; ++i) {
+        const u32 s = scratch[i];
+        off += (i < w) ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
 // Whole-wave histogram update: lanes with `active` add bin `b`.  The most common bins
 // are peeled with one LDS atomic each (two rounds), the rest go one by one.
+#ifndef DRTOPK_HIST_PEEL
+#define DRTOPK_HIST_PEEL 2
+#endif
 __device__ __forceinline__ void wave_hist_add(u32* lhist, bool active, u32 b) {
     const int lane = threadIdx.x & (WAVE - 1);
     bool rem = active;
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
+    for (int it = 0; it < DRTOPK_HIST_PEEL; ++it) {
         const u64 act = __ballot(rem);
         if (!act) return;
         const int src = __ffsll((unsigned long long)act) - 1;

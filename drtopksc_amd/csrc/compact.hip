@@ -85,15 +85,15 @@ constexpr u32 STAGE = 4096;
 
 template <bool IDX, bool HIST2>
 __device__ __forceinline__ void flush_stage(const u32* sk, const u32* si, u32 cnt, u32* __restrict__ gk,
-                                            u32* __restrict__ gi, u64 gpos, u32* lh, int hshift2) {
+                                            u32* __restrict__ gi, u32* lh, int hshift2) {
     for (u32 b = 0; b < cnt; b += WG) {  // uniform bound: every lane reaches wave_hist_add
         const u32 i = b + threadIdx.x;
         const bool act = i < cnt;
         u32 v = 0;
         if (act) {
             v = sk[i];
-            gk[gpos + i] = v;
-            if (IDX) gi[gpos + i] = si[i];
+            gk[i] = v;
+            if (IDX) gi[i] = si[i];
         }
         if (HIST2) wave_hist_add(lh, act, v >> hshift2);
     }
@@ -218,7 +218,10 @@ __device__ __forceinline__ void h2_flush(u32* lh, u32* ghist2, int bins2) {
 }
 
 // Chunks per segment: enough segments to fill the chip (~2048), at most 16 chunks each.
-u64 chunks_per_segment(u64 nchunks) { return std::max<u64>(1, std::min<u64>(16, nchunks / 2048)); }
+#ifndef DRTOPK_CPS_MAX
+#define DRTOPK_CPS_MAX 16
+#endif
+u64 chunks_per_segment(u64 nchunks) { return std::max<u64>(1, std::min<u64>(DRTOPK_CPS_MAX, nchunks / 2048)); }
 
 // ---------------------------------------------------------------- default-path concatenation
 // alpha == 1: one subrange id per key pair; the plain loop below.
@@ -295,7 +298,8 @@ __global__ __launch_bounds__(WG) void k_gather_small(const u32* __restrict__ key
                 }
             }
             __syncthreads();
-            flush_stage<IDX, HIST2>(sk, si, (u32)tot, skeys, sidx, s * segcap + run, lh, hshift2);
+            flush_stage<IDX, HIST2>(sk, si, (u32)tot, skeys + (s * segcap + run),
+                                    IDX ? sidx + (s * segcap + run) : nullptr, lh, hshift2);
             __syncthreads();
             run += tot;
         }
@@ -317,25 +321,26 @@ struct GatherKeys {
     u32x4 v[TE / 4];
 };
 
-__device__ __forceinline__ void gather_load_ids(const u32* __restrict__ sel_ids, u64 V, int alpha, u64 ch, GatherIds& g) {
-    const u64 v0 = ch * TT + (u64)threadIdx.x * TE;
+// Index math is 32-bit: n < 2^31, so key ids, subrange ids and virtual positions fit in u32.
+__device__ __forceinline__ void gather_load_ids(const u32* __restrict__ sel_ids, u32 V, int alpha, u32 ch, GatherIds& g) {
+    const u32 v0 = ch * (u32)TT + threadIdx.x * TE;
 #pragma unroll
     for (int q = 0; q < TE / 4; ++q) {
-        const u64 vp = v0 + 4 * q;
+        const u32 vp = v0 + 4 * q;
         g.v[q] = vp < V ? sel_ids[vp >> alpha] : 0u;
     }
 }
 
-__device__ __forceinline__ void gather_load_keys(const u32* __restrict__ keys, u64 n, u64 V, int alpha, int vec_ok,
-                                                 u64 ch, const GatherIds& g, GatherKeys& k) {
-    const u64 v0 = ch * TT + (u64)threadIdx.x * TE, Lm = (1ull << alpha) - 1;
+__device__ __forceinline__ void gather_load_keys(const u32* __restrict__ keys, u32 n, u32 V, int alpha, int vec_ok,
+                                                 u32 ch, const GatherIds& g, GatherKeys& k) {
+    const u32 v0 = ch * (u32)TT + threadIdx.x * TE, Lm = (1u << alpha) - 1;
 #pragma unroll
     for (int q = 0; q < TE / 4; ++q) {
-        const u64 vp = v0 + 4 * q;
-        const u64 gq = ((u64)g.v[q] << alpha) + (vp & Lm);
+        const u32 vp = v0 + 4 * q;
+        const u32 gq = (g.v[q] << alpha) + (vp & Lm);
         u32x4 r = {0, 0, 0, 0};
         if (vp < V) {
-            if (vec_ok && gq + 3 < n) {
+            if (vec_ok && gq + 4 <= n) {
                 r = *(const u32x4*)(keys + gq);
             } else {
                 if (gq + 0 < n) r.x = keys[gq + 0];
@@ -349,68 +354,72 @@ __device__ __forceinline__ void gather_load_keys(const u32* __restrict__ keys, u
 }
 
 template <class K, bool HIST2, bool IDX>
-__global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64 n, int alpha, int filter, int vec_ok,
-                                               const u32* __restrict__ sel_ids, u64 nsel, u64 cps, Ctrl* ctrl,
+__global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64 n64, int alpha, int filter, int vec_ok,
+                                               const u32* __restrict__ sel_ids, u64 nsel, u64 cps64, Ctrl* ctrl,
                                                u32* __restrict__ skeys, u32* __restrict__ sidx, u32* __restrict__ cnt,
                                                u32* ghist2, int hshift2, int bins2) {
-    __shared__ u64 scratch[WG / WAVE + 1];
+    __shared__ u32 scratch[WG / WAVE];
     __shared__ u32 lh[HIST2 ? 256 : 1];
     __shared__ u32 sk[STAGE];
     __shared__ u32 si[IDX ? STAGE : 1];
     h2_init<HIST2>(lh, bins2);
     const u32 thr = ctrl->first.result;
-    const u64 Lm = (1ull << alpha) - 1;
-    const u64 V = nsel << alpha;
-    const u64 nchunks = (V + TT - 1) / TT;
-    const u64 nseg = (nchunks + cps - 1) / cps;
-    const u64 segcap = cps * TT;
-    for (u64 s = blockIdx.x; s < nseg; s += gridDim.x) {
-        u64 run = 0;
-        const u64 cb = s * cps, ce = std::min<u64>((s + 1) * cps, nchunks);
+    const u32 n = (u32)n64;
+    const u32 Lm = (1u << alpha) - 1;
+    const u32 V = (u32)(nsel << alpha);
+    const u32 cps = (u32)cps64;
+    const u32 nchunks = (V + (u32)TT - 1) / (u32)TT;
+    const u32 nseg = (nchunks + cps - 1) / cps;
+    const u64 segcap = (u64)cps * TT;
+    for (u32 s = blockIdx.x; s < nseg; s += gridDim.x) {
+        u32 run = 0;
+        const u32 cb = s * cps, ce = min((s + 1) * cps, nchunks);
         GatherIds ia, ib;
         GatherKeys ka;
         gather_load_ids(sel_ids, V, alpha, cb, ia);
         if (cb + 1 < ce) gather_load_ids(sel_ids, V, alpha, cb + 1, ib);
         gather_load_keys(keys, n, V, alpha, vec_ok, cb, ia, ka);
-        for (u64 ch = cb; ch < ce; ++ch) {
+        for (u32 ch = cb; ch < ce; ++ch) {
             GatherIds ic;
             GatherKeys kb;
             if (ch + 2 < ce) gather_load_ids(sel_ids, V, alpha, ch + 2, ic);
             if (ch + 1 < ce) gather_load_keys(keys, n, V, alpha, vec_ok, ch + 1, ib, kb);
-            const u64 v0 = ch * TT + (u64)threadIdx.x * TE;
+            const u32 v0 = ch * (u32)TT + threadIdx.x * TE;
             u32 kv[TE];
             u32 take = 0;  // bit j: element j passes
 #pragma unroll
             for (int q = 0; q < TE / 4; ++q) {
-                const u64 vp = v0 + 4 * q;
-                const u64 gq = ((u64)ia.v[q] << alpha) + (vp & Lm);
+                const u32 vp = v0 + 4 * q;
+                const u32 gq = (ia.v[q] << alpha) + (vp & Lm);
+                const bool gwhole = vp + 4 <= V && gq + 4 <= n;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const u32 o = K::to_ord(ka.v[q][c]);
                     kv[4 * q + c] = o;
-                    const bool ok = vp + c < V && gq + c < n;
+                    const bool ok = gwhole || (vp + c < V && gq + c < n);
                     if (ok && (!filter || o >= thr)) take |= 1u << (4 * q + c);
                 }
             }
-            u64 tot;
-            u32 slot = (u32)block_excl_scan_u64<WG>((u64)__popc(take), scratch, &tot);
+            u32 tot;
+            u32 slot = block_excl_scan_u32<WG>((u32)__popc(take), scratch, &tot);
 #pragma unroll
             for (int j = 0; j < TE; ++j) {
                 if (take & (1u << j)) {
                     sk[slot] = kv[j];
-                    if (IDX) si[slot] = (u32)(((u64)ia.v[j >> 2] << alpha) + ((v0 + j) & Lm));
+                    if (IDX) si[slot] = (ia.v[j >> 2] << alpha) + ((v0 + j) & Lm);
                     ++slot;
                 }
             }
             __syncthreads();
-            flush_stage<IDX, HIST2>(sk, si, (u32)tot, skeys, sidx, s * segcap + run, lh, hshift2);
+            const u64 gpos = (u64)s * segcap + run;
+            flush_stage<IDX, HIST2>(sk, si, tot, skeys + gpos, IDX ? sidx + gpos : nullptr, lh, hshift2);
             __syncthreads();
             run += tot;
             ia = ib;
             ib = ic;
             ka = kb;
         }
-        if (threadIdx.x == 0) cnt[s] = (u32)run;
+        if (threadIdx.x == 0) cnt[s] = run;
     }
     h2_flush<HIST2>(lh, ghist2, bins2);
 }
@@ -464,9 +473,9 @@ constexpr int BSPT = DRTOPK_BG_BSPT;  // subranges per thread of the beta gather
 constexpr u32 BSTAGE = DRTOPK_BG_STAGE;  // staged survivors per chunk (more: direct stores)
 constexpr u32 KBUF = DRTOPK_BG_STAGE;  // keys of listed subranges held in LDS
 
-template <int BETA>
+template <int BETA, bool WHOLE = false>
 __device__ __forceinline__ void load_deleg(const u32* __restrict__ dvals, u64 s0, u64 S, u32 (&d)[BSPT][BETA]) {
-    if (s0 + BSPT <= S) {
+    if (WHOLE || s0 + BSPT <= S) {
         const u32x4* p = (const u32x4*)(dvals + s0 * BETA);  // BSPT*BETA words, 16-byte aligned
         u32 w[BSPT * BETA];
 #pragma unroll
@@ -489,11 +498,11 @@ __device__ __forceinline__ void load_deleg(const u32* __restrict__ dvals, u64 s0
 // Packed in-subrange offsets of the first beta-1 delegates of the thread's BSPT subranges
 // (u8 / u16 / u32 per subrange for beta 2 / 3 / 4): BSPT * {1,2,4} bytes at an offset that is a
 // multiple of that size, read with one aligned load.
-template <int BETA>
+template <int BETA, bool WHOLE = false>
 __device__ __forceinline__ void load_doff(const void* __restrict__ doff, u64 s0, u64 S, u32 (&pk)[BSPT]) {
     static_assert(BSPT == 4 || BSPT == 8, "packed-offset loads assume 4 or 8 subranges per thread");
     constexpr int EB = BETA == 2 ? 1 : (BETA == 3 ? 2 : 4);  // bytes per subrange
-    if (s0 + BSPT <= S) {
+    if (WHOLE || s0 + BSPT <= S) {
         const unsigned char* p = (const unsigned char*)doff + s0 * EB;
         if constexpr (EB * BSPT == 4) {
             const u32 q = *(const u32*)p;
@@ -530,16 +539,20 @@ __device__ __forceinline__ void load_doff(const void* __restrict__ doff, u64 s0,
     }
 }
 
-#ifndef DRTOPK_BG_WAVES
-#define DRTOPK_BG_WAVES 5  // fits 96 VGPRs without spilling; 6 and 8 spill and run slower
+#ifndef DRTOPK_BG_PF
+#define DRTOPK_BG_PF 0
 #endif
-template <class K, int BETA, bool POS, bool HIST2>
-__global__ __launch_bounds__(WG, DRTOPK_BG_WAVES) void k_beta_gather(const u32* __restrict__ keys, u64 n, int alpha, int filter,
+// The kernel is bound by instruction issue rather than by HBM (about 1.3 GB moved per launch at
+// k = 2^24, alpha 4), so everything below works in 32 bits: n < 2^31 keeps key and subrange ids,
+// and chunk-local offsets, inside u32.  A chunk whose subranges are all whole (every chunk but
+// the last) skips the per-subrange and per-key bounds checks.
+template <class K, int BETA, bool POS, bool HIST2, int WAVES>
+__global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict__ keys, u64 n64, int alpha, int filter,
                                                     const u32* __restrict__ dvals, const void* __restrict__ doff,
-                                                    u64 cps, Ctrl* ctrl, u32* __restrict__ skeys,
+                                                    u64 cps64, Ctrl* ctrl, u32* __restrict__ skeys,
                                                     u32* __restrict__ sidx, u32* __restrict__ cnt, u32* ghist2,
                                                     int hshift2, int bins2) {
-    __shared__ u64 scratch[WG / WAVE + 1];
+    __shared__ u32 scratch[WG / WAVE];
     __shared__ u32 lh[HIST2 ? 256 : 1];
     __shared__ u64 red[4][WG / WAVE];
     __shared__ u32 sk[BSTAGE];
@@ -548,48 +561,75 @@ __global__ __launch_bounds__(WG, DRTOPK_BG_WAVES) void k_beta_gather(const u32* 
     __shared__ u32 spos[BETA_TILE_SUBS];   // their kept-key counts, then their first output slot
     __shared__ __align__(16) u32 kbuf[KBUF];
     h2_init<HIST2>(lh, bins2);
+    const u32 n = (u32)n64;
     const u32 thr = ctrl->first.result;
-    const u64 L = 1ull << alpha;
-    const u64 S = (n + L - 1) >> alpha;
-    const u64 nchunks = (S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS;
-    const u64 nseg = (nchunks + cps - 1) / cps;
-    const u64 segcap = cps * (BETA_TILE_SUBS << alpha);
+    const u32 L = 1u << alpha;
+    const u32 S = (u32)((n64 + L - 1) >> alpha);
+    const u32 cps = (u32)cps64;
+    const u32 nchunks = (S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS;
+    const u32 nseg = (nchunks + cps - 1) / cps;
+    const u64 segcap = (u64)cps * ((u64)BETA_TILE_SUBS << alpha);
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const int G = alpha >= 2 ? (1 << (alpha - 2)) : 1;  // lanes per listed subrange (alpha <= 8)
     const int gl = lane & (G - 1), grp = lane / G, per_wave = WAVE / G;
-    u64 nsel = 0, nlon = 0, nlel = 0, ndrop = 0;
-    for (u64 sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
-        u64 run = 0;
-        const u64 ch_end = std::min<u64>((sg + 1) * cps, nchunks);
-        for (u64 ch = sg * cps; ch < ch_end; ++ch) {
-            const u64 c0 = ch * BETA_TILE_SUBS;
-            const u64 s0 = c0 + (u64)threadIdx.x * BSPT;
+    const u32 l0 = threadIdx.x * BSPT;  // chunk-local id of the thread's first subrange
+    u32 nsel = 0, nlon = 0, nlel = 0, ndrop = 0;
+#if DRTOPK_BG_PF
+    u32 dn[BSPT][BETA], pkn[BSPT];
+#endif
+    for (u32 sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+        u32 run = 0;
+        const u32 ch_end = min((sg + 1) * cps, nchunks);
+        for (u32 ch = sg * cps; ch < ch_end; ++ch) {
+            const u32 c0 = ch * BETA_TILE_SUBS;  // first subrange of the chunk
+            const u32 kb = c0 << alpha;          // its first key
+            const bool full = ((u64)(c0 + BETA_TILE_SUBS) << alpha) <= n64;  // uniform
             u32 d[BSPT][BETA], pk[BSPT];
-            load_deleg<BETA>(dvals, s0, S, d);
-            if (POS) load_doff<BETA>(doff, s0, S, pk);
+#if DRTOPK_BG_PF
+            if (ch == sg * cps) {
+                load_deleg<BETA>(dvals, c0 + l0, S, dn);
+                if (POS) load_doff<BETA>(doff, c0 + l0, S, pkn);
+            }
+#pragma unroll
+            for (int j = 0; j < BSPT; ++j) {
+                pk[j] = pkn[j];
+#pragma unroll
+                for (int b = 0; b < BETA; ++b) d[j][b] = dn[j][b];
+            }
+            if (ch + 1 < ch_end) {  // the next chunk's delegates, in flight during this one
+                load_deleg<BETA>(dvals, c0 + BETA_TILE_SUBS + l0, S, dn);
+                if (POS) load_doff<BETA>(doff, c0 + BETA_TILE_SUBS + l0, S, pkn);
+            }
+#else
+            if (full) {
+                load_deleg<BETA, true>(dvals, c0 + l0, S, d);
+                if (POS) load_doff<BETA, true>(doff, c0 + l0, S, pk);
+            } else {
+                load_deleg<BETA>(dvals, c0 + l0, S, d);
+                if (POS) load_doff<BETA>(doff, c0 + l0, S, pk);
+            }
+#endif
             int kind[BSPT];  // 0 none, 1 selected (its keys), 2 lonely (its delegates)
-            u32 mj[BSPT];    // lonely: delegates written (with FILTER only those >= threshold)
+            u32 mj[BSPT];    // lonely: delegates written (with filter bit 2 only those >= threshold)
             u32 ns_t = 0;
 #pragma unroll
             for (int j = 0; j < BSPT; ++j) {
-                const u64 s = s0 + j;
+                const u32 s = c0 + l0 + j;
                 kind[j] = 0;
                 mj[j] = 0;
-                if (s >= S || d[j][0] < thr) continue;
-                const u64 len = std::min<u64>(L, n - (s << alpha));
-                const int m = (int)std::min<u64>(BETA, len);
+                if ((!full && s >= S) || d[j][0] < thr) continue;
+                const int m = full ? BETA : (int)min<u32>(BETA, n - (s << alpha));
                 bool all = true;
 #pragma unroll
                 for (int b = 0; b < BETA; ++b) all = all && (b >= m || d[j][b] >= thr);
                 mj[j] = (u32)m;
                 if (all) {
                     kind[j] = 1;
-                    ++nsel;
                     ++ns_t;
                 } else {
                     kind[j] = 2;
                     ++nlon;
-                    nlel += (u64)m;
+                    nlel += (u32)m;
                     if (filter & 2) {
                         // a lonely delegate below the threshold is below the k-th largest key:
                         // counted in the concatenation's size, not written (radix family; the
@@ -599,22 +639,22 @@ __global__ __launch_bounds__(WG, DRTOPK_BG_WAVES) void k_beta_gather(const u32* 
 #pragma unroll
                         for (int b = 0; b < BETA; ++b) ge += (b < m && d[j][b] >= thr) ? 1u : 0u;
                         mj[j] = ge;
-                        ndrop += (u64)(m - (int)ge);
+                        ndrop += (u32)m - ge;
                     }
                 }
             }
+            nsel += ns_t;
             // (1) list the selected subranges
-            u64 tot_s;
-            const u64 ex_s = block_excl_scan_u64<WG>((u64)ns_t, scratch, &tot_s);
+            u32 ns;
+            const u32 ex_s = block_excl_scan_u32<WG>(ns_t, scratch, &ns);
             {
-                u32 q = (u32)ex_s;
+                u32 q = ex_s;
 #pragma unroll
                 for (int j = 0; j < BSPT; ++j)
-                    if (kind[j] == 1) slist[q++] = (uint16_t)(threadIdx.x * BSPT + j);
+                    if (kind[j] == 1) slist[q++] = (uint16_t)(l0 + j);
             }
             __syncthreads();
-            const u32 ns = (u32)tot_s;
-            const bool inl = alpha >= 2 && ((u64)ns << alpha) <= KBUF;  // uniform
+            const bool inl = alpha >= 2 && (ns << alpha) <= KBUF;  // uniform
             if (inl) {
                 const u32 ngran = (ns << alpha) >> 2;
                 const int lg = alpha - 2;
@@ -625,8 +665,8 @@ __global__ __launch_bounds__(WG, DRTOPK_BG_WAVES) void k_beta_gather(const u32* 
                         const u32 x = x0 + u * WG + threadIdx.x;
                         r[u] = u32x4{0, 0, 0, 0};
                         if (x < ngran) {
-                            const u64 e = ((c0 + slist[x >> lg]) << alpha) + 4ull * (x & ((1u << lg) - 1));
-                            if (e + 4 <= n) r[u] = *(const u32x4*)(keys + e);
+                            const u32 e = kb + ((u32)slist[x >> lg] << alpha) + ((x & ((1u << lg) - 1)) << 2);
+                            if (full || e + 4 <= n) r[u] = *(const u32x4*)(keys + e);
                             else
                                 for (int jj = 0; jj < 4; ++jj)
                                     if (e + jj < n) r[u][jj] = keys[e + jj];
@@ -644,23 +684,24 @@ __global__ __launch_bounds__(WG, DRTOPK_BG_WAVES) void k_beta_gather(const u32* 
                 }
                 __syncthreads();
             }
-            // ordered keys and valid mask of granule gl (4 keys) of listed subrange i
-            auto fetch4 = [&](u32 i, u32* kv, u64* e_out) -> u32 {
-                const u64 e = ((c0 + slist[i]) << alpha) + 4 * (u64)gl;
+            // ordered keys and kept mask of granule gl (4 keys) of listed subrange i
+            auto fetch4 = [&](u32 i, u32* kv, u32* e_out) -> u32 {
+                const u32 e = kb + ((u32)slist[i] << alpha) + 4u * (u32)gl;
                 *e_out = e;
-                u32 okm = 0;
+                u32 okm = 0xF;
                 if (inl) {
-                    const u32x4 v = *(const u32x4*)(kbuf + ((u64)i << alpha) + 4 * gl);
+                    const u32x4 v = *(const u32x4*)(kbuf + (i << alpha) + 4 * gl);
                     kv[0] = v.x; kv[1] = v.y; kv[2] = v.z; kv[3] = v.w;
-                    for (int jj = 0; jj < 4; ++jj) okm |= (e + jj < n ? 1u : 0u) << jj;
-                } else if (e + 4 <= n) {
+                    if (!full)
+                        for (int jj = 0; jj < 4; ++jj)
+                            if (e + jj >= n) okm &= ~(1u << jj);
+                } else if (full || e + 4 <= n) {
                     const u32x4 r = *(const u32x4*)(keys + e);
                     kv[0] = K::to_ord(r.x); kv[1] = K::to_ord(r.y); kv[2] = K::to_ord(r.z); kv[3] = K::to_ord(r.w);
-                    okm = 0xF;
                 } else {
                     for (int jj = 0; jj < 4; ++jj) {
                         kv[jj] = e + jj < n ? K::to_ord(keys[e + jj]) : 0u;
-                        okm |= (e + jj < n ? 1u : 0u) << jj;
+                        if (e + jj >= n) okm &= ~(1u << jj);
                     }
                 }
                 if (filter)
@@ -674,8 +715,7 @@ __global__ __launch_bounds__(WG, DRTOPK_BG_WAVES) void k_beta_gather(const u32* 
                     const u32 i = i0 + grp;
                     u32 c = 0;
                     if (i < ns) {
-                        u32 kv[4];
-                        u64 e;
+                        u32 kv[4], e;
                         c = (u32)__popc(fetch4(i, kv, &e));
                     }
                     for (int o = 1; o < G; o <<= 1) c += (u32)__shfl_xor((int)c, o, WAVE);
@@ -683,10 +723,10 @@ __global__ __launch_bounds__(WG, DRTOPK_BG_WAVES) void k_beta_gather(const u32* 
                 }
             } else {
                 for (u32 i = threadIdx.x; i < ns; i += WG) {
-                    const u64 base = (c0 + slist[i]) << alpha;
-                    const u64 len = std::min<u64>(L, n - base);
+                    const u32 base = kb + ((u32)slist[i] << alpha);
+                    const u32 len = min(L, n - base);
                     u32 c = 0;
-                    for (u64 e = base; e < base + len; ++e) c += (!filter || K::to_ord(keys[e]) >= thr) ? 1u : 0u;
+                    for (u32 e = base; e < base + len; ++e) c += (!filter || K::to_ord(keys[e]) >= thr) ? 1u : 0u;
                     spos[i] = c;
                 }
             }
@@ -694,49 +734,65 @@ __global__ __launch_bounds__(WG, DRTOPK_BG_WAVES) void k_beta_gather(const u32* 
             // (3) output slots
             u32 c = 0;
             {
-                u32 q = (u32)ex_s;
+                u32 q = ex_s;
 #pragma unroll
                 for (int j = 0; j < BSPT; ++j) {
                     if (kind[j] == 1) c += spos[q++];
                     else if (kind[j] == 2) c += mj[j];
                 }
             }
-            u64 tot;
-            const u64 ex = block_excl_scan_u64<WG>((u64)c, scratch, &tot);
+            u32 tot;
+            const u32 ex = block_excl_scan_u32<WG>(c, scratch, &tot);
             const bool staged = tot <= BSTAGE;  // uniform
-            const u64 gbase = sg * segcap + run;
-            auto put = [&](u64 slot, u32 o, u32 gidx) {
+            const u64 gbase = (u64)sg * segcap + run;
+            u32* __restrict__ gk = skeys + gbase;
+            u32* __restrict__ gi = POS ? sidx + gbase : nullptr;
+            auto put = [&](u32 slot, u32 o, u32 gidx) {
                 if (staged) {
                     sk[slot] = o;
                     if (POS) si[slot] = gidx;
                 } else {
-                    skeys[gbase + slot] = o;
-                    if (POS) sidx[gbase + slot] = gidx;
+                    gk[slot] = o;
+                    if (POS) gi[slot] = gidx;
                     if (HIST2) atomicAdd(&lh[o >> hshift2], 1u);
                 }
             };
-            {
-                u64 slot = ex;
-                u32 q = (u32)ex_s;
+            if (BETA == 2 && (filter & 2)) {
+                // beta 2 with lonely delegates below the threshold dropped: a lonely subrange
+                // writes exactly its first delegate (the second is below the threshold)
+                u32 slot = ex;
+                u32 q = ex_s;
+#pragma unroll
+                for (int j = 0; j < BSPT; ++j) {
+                    const bool sel = kind[j] == 1, lon = kind[j] == 2;
+                    u32 cj = lon ? 1u : 0u;
+                    if (sel) {
+                        cj = spos[q];
+                        spos[q] = slot;
+                    }
+                    if (lon) put(slot, d[j][0], kb + ((l0 + j) << alpha) + (POS ? (pk[j] & 0xFFu) : 0u));
+                    slot += cj;
+                    q += sel ? 1u : 0u;
+                }
+            } else {
+                u32 slot = ex;
+                u32 q = ex_s;
 #pragma unroll
                 for (int j = 0; j < BSPT; ++j) {
                     if (kind[j] == 1) {
                         const u32 cj = spos[q];
-                        spos[q] = (u32)slot;
+                        spos[q] = slot;
                         slot += cj;
                         ++q;
                     } else if (kind[j] == 2) {
-                        const u64 s = s0 + j;
-                        const u64 base = s << alpha;
-                        const int m = (int)mj[j];
+                        const u32 base = kb + ((l0 + j) << alpha);
                         u32 dv[BETA], p[BETA];
                         const u32 packed = POS ? pk[j] : 0u;
 #pragma unroll
                         for (int b = 0; b < BETA; ++b) {
                             dv[b] = d[j][b];
-                            const bool ge = b < m && dv[b] >= thr;  // only these can reach the output
-                            p[b] = (POS && ge && b < BETA - 1) ? (u32)(base + ((packed >> (8 * b)) & 0xFFu))
-                                                               : 0xFFFFFFFFu;
+                            const bool ge = b < (int)mj[j] && dv[b] >= thr;  // only these can reach the output
+                            p[b] = (POS && ge && b < BETA - 1) ? base + ((packed >> (8 * b)) & 0xFFu) : 0xFFFFFFFFu;
                         }
                         // delegates >= threshold in index order, then the rest (never selected later)
 #pragma unroll
@@ -758,8 +814,7 @@ __global__ __launch_bounds__(WG, DRTOPK_BG_WAVES) void k_beta_gather(const u32* 
             if (alpha >= 2) {
                 for (u32 i0 = (u32)wv * per_wave; i0 < ns; i0 += (WG / WAVE) * per_wave) {
                     const u32 i = i0 + grp;
-                    u32 kv[4] = {0, 0, 0, 0}, okm = 0;
-                    u64 e = 0;
+                    u32 kv[4] = {0, 0, 0, 0}, okm = 0, e = 0;
                     if (i < ns) okm = fetch4(i, kv, &e);
                     const u32 c4 = (u32)__popc(okm);
                     u32 incl = c4;
@@ -768,39 +823,37 @@ __global__ __launch_bounds__(WG, DRTOPK_BG_WAVES) void k_beta_gather(const u32* 
                         if (gl >= o) incl += tt;
                     }
                     if (i < ns) {
-                        u64 slot = spos[i] + (incl - c4);
+                        u32 slot = spos[i] + (incl - c4);
                         for (int jj = 0; jj < 4; ++jj)
-                            if ((okm >> jj) & 1u) put(slot++, kv[jj], (u32)(e + jj));
+                            if ((okm >> jj) & 1u) put(slot++, kv[jj], e + jj);
                     }
                 }
             } else {
                 for (u32 i = threadIdx.x; i < ns; i += WG) {
-                    const u64 base = (c0 + slist[i]) << alpha;
-                    const u64 len = std::min<u64>(L, n - base);
-                    u64 slot = spos[i];
-                    for (u64 e = base; e < base + len; ++e) {
+                    const u32 base = kb + ((u32)slist[i] << alpha);
+                    const u32 len = min(L, n - base);
+                    u32 slot = spos[i];
+                    for (u32 e = base; e < base + len; ++e) {
                         const u32 o = K::to_ord(keys[e]);
-                        if (!filter || o >= thr) put(slot++, o, (u32)e);
+                        if (!filter || o >= thr) put(slot++, o, e);
                     }
                 }
             }
             __syncthreads();
-            if (staged) flush_stage<POS, HIST2>(sk, si, (u32)tot, skeys, sidx, gbase, lh, hshift2);
+            if (staged) flush_stage<POS, HIST2>(sk, si, tot, gk, gi, lh, hshift2);
             __syncthreads();
             run += tot;
         }
-        if (threadIdx.x == 0) cnt[sg] = (u32)run;
+        if (threadIdx.x == 0) cnt[sg] = run;
     }
     {
         const int w = threadIdx.x >> 6;
+        u64 a4[4] = {nsel, nlon, nlel, ndrop};
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            nsel += shfl_xor_u64(nsel, o);
-            nlon += shfl_xor_u64(nlon, o);
-            nlel += shfl_xor_u64(nlel, o);
-            ndrop += shfl_xor_u64(ndrop, o);
+        for (int q = 0; q < 4; ++q) {
+            a4[q] = wave_sum_u64(a4[q]);
+            if (lane == 0) red[q][w] = a4[q];
         }
-        if (lane == 0) { red[0][w] = nsel; red[1][w] = nlon; red[2][w] = nlel; red[3][w] = ndrop; }
         __syncthreads();
         if (threadIdx.x == 0) {
             u64 a = 0, b = 0, cc = 0, dd = 0;
@@ -834,10 +887,18 @@ static hipError_t launch_beta_t(const BetaGatherArgs& a, hipStream_t st) {
 #define B_ARGS a.keys, a.n, a.alpha, a.filter, a.dvals, a.doff, cps, a.ctrl, a.skeys, a.sidx, a.cnt, a.ghist2, \
                a.hshift2, a.bins2
     const bool pos = a.sidx != nullptr, h = a.ghist2 != nullptr;
-    if (pos && h) hipLaunchKernelGGL((k_beta_gather<K, BETA, true, true>), dim3(grid), dim3(WG), 0, st, B_ARGS);
-    else if (pos) hipLaunchKernelGGL((k_beta_gather<K, BETA, true, false>), dim3(grid), dim3(WG), 0, st, B_ARGS);
-    else if (h) hipLaunchKernelGGL((k_beta_gather<K, BETA, false, true>), dim3(grid), dim3(WG), 0, st, B_ARGS);
-    else hipLaunchKernelGGL((k_beta_gather<K, BETA, false, false>), dim3(grid), dim3(WG), 0, st, B_ARGS);
+    // 32-key subranges (alpha 5) run faster at 8 waves per SIMD (64 VGPRs); at alpha <= 4 the
+    // random 16-byte subrange reads are the limit and 5 waves do better
+#define B_LAUNCH(W)                                                                                              \
+    do {                                                                                                         \
+        if (pos && h) hipLaunchKernelGGL((k_beta_gather<K, BETA, true, true, W>), dim3(grid), dim3(WG), 0, st, B_ARGS);   \
+        else if (pos) hipLaunchKernelGGL((k_beta_gather<K, BETA, true, false, W>), dim3(grid), dim3(WG), 0, st, B_ARGS);  \
+        else if (h) hipLaunchKernelGGL((k_beta_gather<K, BETA, false, true, W>), dim3(grid), dim3(WG), 0, st, B_ARGS);    \
+        else hipLaunchKernelGGL((k_beta_gather<K, BETA, false, false, W>), dim3(grid), dim3(WG), 0, st, B_ARGS);          \
+    } while (0)
+    if (a.alpha >= 5) B_LAUNCH(8);
+    else B_LAUNCH(5);
+#undef B_LAUNCH
 #undef B_ARGS
     return hipGetLastError();
 }
@@ -881,6 +942,18 @@ __device__ __forceinline__ u64 seg_len(const Seg& g, u64 s) {
     return DIRECT ? std::min<u64>(g.segcap, g.n - s * g.segcap) : (u64)g.cnt[s];
 }
 
+// Counts of keys > K and == K of thread-owned runs of TE keys; `whole` (uniform) skips the
+// per-key bounds checks of a chunk that lies inside the segment.
+template <int N>
+__device__ __forceinline__ void count_gt_eq(const u32 (&v)[N], u32 K0, bool whole, u32 nvalid, u32& gt, u32& eq) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const bool in = whole || (u32)j < nvalid;
+        gt += (in && v[j] > K0) ? 1u : 0u;
+        eq += (in && v[j] == K0) ? 1u : 0u;
+    }
+}
+
 template <class K, bool DIRECT>
 __global__ __launch_bounds__(WG) void k_extract_count(Seg g, SelState* sel, u64* __restrict__ cnt) {
     __shared__ u64 scratch[WG / WAVE + 1];
@@ -891,14 +964,10 @@ __global__ __launch_bounds__(WG) void k_extract_count(Seg g, SelState* sel, u64*
         u32 gt = 0, eq = 0;
         for (u64 c0 = 0; c0 < len; c0 += TT) {
             const u64 e0 = c0 + (u64)threadIdx.x * TE;
+            const bool whole = c0 + TT <= len;
             u32 v[TE];
             load_te<K, DIRECT>(base + e0, e0, len, v);
-#pragma unroll
-            for (int j = 0; j < TE; ++j) {
-                const bool in = e0 + j < len;
-                gt += in && v[j] > K0;
-                eq += in && v[j] == K0;
-            }
+            count_gt_eq(v, K0, whole, e0 < len ? (u32)std::min<u64>(len - e0, TE) : 0u, gt, eq);
         }
         u64 tot;
         block_excl_scan_u64<WG>(pack2(gt, eq), scratch, &tot);
@@ -906,11 +975,14 @@ __global__ __launch_bounds__(WG) void k_extract_count(Seg g, SelState* sel, u64*
     }
 }
 
+// Per chunk of TT keys the outputs are placed relative to the chunk's first slot P0 in 32 bits:
+// with lg / lq the keys > K / == K before a key inside the chunk and rem the ties still to take
+// when the chunk starts, a key > K goes to lg + min(lq, rem) and a key == K with lq < rem to lg + lq.
 template <class K, bool DIRECT, bool KEYS, bool IDX>
 __global__ __launch_bounds__(WG) void k_extract_write(Seg g, SelState* sel, u64 k, const u64* __restrict__ off,
                                                       u32* __restrict__ out_keys, i64* __restrict__ out_idx,
                                                       Ctrl* ctrl) {
-    __shared__ u64 scratch[WG / WAVE + 1];
+    __shared__ u32 scratch[WG / WAVE];
     __shared__ u32 sk[KEYS ? STAGE : 1];
     __shared__ u32 si[IDX ? STAGE : 1];
     const u32 K0 = sel->result;
@@ -923,54 +995,49 @@ __global__ __launch_bounds__(WG) void k_extract_write(Seg g, SelState* sel, u64 
         u64 run_g = unpack_hi(o), run_q = unpack_lo(o);
         for (u64 c0 = 0; c0 < len; c0 += TT) {
             const u64 e0 = c0 + (u64)threadIdx.x * TE;
+            const bool whole = c0 + TT <= len;  // uniform
+            const u32 nvalid = e0 < len ? (u32)std::min<u64>(len - e0, TE) : 0u;
             u32 v[TE];
             u32 vi[(IDX && !DIRECT) ? TE : 1];
             load_te<K, DIRECT>(base + e0, e0, len, v);
             // the concatenation's indices, read with the keys (16-byte loads) rather than one by one later
             if (IDX && !DIRECT) load_te<KeyU32, false>(g.idx + s * g.segcap + e0, e0, len, vi);
             u32 gt = 0, eq = 0;
-#pragma unroll
-            for (int j = 0; j < TE; ++j) {
-                const bool in = e0 + j < len;
-                gt += in && v[j] > K0;
-                eq += in && v[j] == K0;
-            }
-            u64 tot;
-            const u64 ex = block_excl_scan_u64<WG>(pack2(gt, eq), scratch, &tot);
-            // the chunk's outputs occupy [P0, P1) of the top-k arrays
+            count_gt_eq(v, K0, whole, nvalid, gt, eq);
+            u32 tot;  // packed (gt << 16) | eq: at most TT = 4096 each
+            const u32 ex = block_excl_scan_u32<WG>((gt << 16) | eq, scratch, &tot);
+            const u64 rem64 = r - std::min<u64>(run_q, r);
+            const u32 rem = (u32)std::min<u64>(rem64, TT);
+            // the chunk's outputs occupy [P0, P0 + c) of the top-k arrays
             const u64 P0 = run_g + std::min<u64>(run_q, r);
-            const u64 P1 = run_g + unpack_hi(tot) + std::min<u64>(run_q + unpack_lo(tot), r);
-            u64 gg = run_g + unpack_hi(ex), q = run_q + unpack_lo(ex);
+            const u32 c = (tot >> 16) + std::min<u32>(tot & 0xFFFFu, rem);
+            u32 lg = ex >> 16, lq = ex & 0xFFFFu;
 #pragma unroll
             for (int j = 0; j < TE; ++j) {
-                if (e0 + j >= len) break;
-                u64 pos = ~0ull;
-                if (v[j] > K0) {
-                    pos = gg + (q < r ? q : r);
-                    ++gg;
-                } else if (v[j] == K0) {
-                    if (q < r) pos = gg + q;
-                    ++q;
-                }
-                if (pos != ~0ull) {
-                    const u32 sl = (u32)(pos - P0);
+                const bool in = whole || (u32)j < nvalid;
+                const bool isg = in && v[j] > K0, ise = in && v[j] == K0;
+                const u32 sl = isg ? lg + min(lq, rem) : lg + lq;
+                if (isg || (ise && lq < rem)) {
                     if (KEYS) sk[sl] = K::from_ord(v[j]);
                     if (IDX) si[sl] = DIRECT ? (u32)(s * g.segcap + e0 + j) : vi[(IDX && !DIRECT) ? j : 0];
                 }
+                lg += isg ? 1u : 0u;
+                lq += ise ? 1u : 0u;
             }
             __syncthreads();
-            if (P1 > k) {
+            if (P0 + c > k) {
                 if (threadIdx.x == 0) atomicOr(&ctrl->error, ERR_CAPACITY);
             } else {
-                const u32 c = (u32)(P1 - P0);
+                u32* __restrict__ ok = KEYS ? out_keys + P0 : nullptr;
+                i64* __restrict__ oi = IDX ? out_idx + P0 : nullptr;
                 for (u32 i = threadIdx.x; i < c; i += WG) {
-                    if (KEYS) out_keys[P0 + i] = sk[i];
-                    if (IDX) out_idx[P0 + i] = (i64)si[i];
+                    if (KEYS) ok[i] = sk[i];
+                    if (IDX) oi[i] = (i64)si[i];
                 }
             }
             __syncthreads();
-            run_g += unpack_hi(tot);
-            run_q += unpack_lo(tot);
+            run_g += tot >> 16;
+            run_q += tot & 0xFFFFu;
         }
     }
 }
