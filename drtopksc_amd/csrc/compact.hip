@@ -82,6 +82,10 @@ __device__ __forceinline__ void load_te(const u32* p, u64 e0, u64 m, u32* v) {
 // chunk's run out with coalesced stores (a thread's own survivors sit at scattered global
 // positions, so storing them directly costs one cache line per lane per store).
 constexpr u32 STAGE = 4096;
+#ifndef DRTOPK_G_STAGE
+#define DRTOPK_G_STAGE 4096
+#endif
+constexpr u32 GSTAGE = DRTOPK_G_STAGE;  // survivors of a default-path gather chunk staged in LDS
 
 template <bool IDX, bool HIST2>
 __device__ __forceinline__ void flush_stage(const u32* sk, const u32* si, u32 cnt, u32* __restrict__ gk,
@@ -322,46 +326,52 @@ struct GatherKeys {
 };
 
 // Index math is 32-bit: n < 2^31, so key ids, subrange ids and virtual positions fit in u32.
+// Loads are issued unconditionally from clamped in-bounds addresses: a load under a per-lane
+// condition makes the compiler wait for it right where the branches join, which would serialize
+// the loads of a thread and defeat the prefetch.  Lanes past the end mask their keys later.
 __device__ __forceinline__ void gather_load_ids(const u32* __restrict__ sel_ids, u32 V, int alpha, u32 ch, GatherIds& g) {
     const u32 v0 = ch * (u32)TT + threadIdx.x * TE;
 #pragma unroll
     for (int q = 0; q < TE / 4; ++q) {
         const u32 vp = v0 + 4 * q;
-        g.v[q] = vp < V ? sel_ids[vp >> alpha] : 0u;
+        g.v[q] = sel_ids[vp < V ? vp >> alpha : 0u];
     }
 }
 
-__device__ __forceinline__ void gather_load_keys(const u32* __restrict__ keys, u32 n, u32 V, int alpha, int vec_ok,
+// VEC: 16-byte aligned keys and n >= 4.  The last granule of the input, when n is not a multiple
+// of 4, is loaded from key 0 here and re-read key by key where it is consumed (gather_fix).
+template <bool VEC>
+__device__ __forceinline__ void gather_load_keys(const u32* __restrict__ keys, u32 n, u32 V, int alpha,
                                                  u32 ch, const GatherIds& g, GatherKeys& k) {
     const u32 v0 = ch * (u32)TT + threadIdx.x * TE, Lm = (1u << alpha) - 1;
 #pragma unroll
     for (int q = 0; q < TE / 4; ++q) {
         const u32 vp = v0 + 4 * q;
         const u32 gq = (g.v[q] << alpha) + (vp & Lm);
-        u32x4 r = {0, 0, 0, 0};
-        if (vp < V) {
-            if (vec_ok && gq + 4 <= n) {
-                r = *(const u32x4*)(keys + gq);
-            } else {
+        if (VEC) {
+            k.v[q] = *(const u32x4*)(keys + ((vp < V && gq + 4 <= n) ? gq : 0u));
+        } else {
+            u32x4 r = {0, 0, 0, 0};
+            if (vp < V) {
                 if (gq + 0 < n) r.x = keys[gq + 0];
                 if (gq + 1 < n) r.y = keys[gq + 1];
                 if (gq + 2 < n) r.z = keys[gq + 2];
                 if (gq + 3 < n) r.w = keys[gq + 3];
             }
+            k.v[q] = r;
         }
-        k.v[q] = r;
     }
 }
 
-template <class K, bool HIST2, bool IDX>
+template <class K, bool HIST2, bool IDX, bool VEC>
 __global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64 n64, int alpha, int filter, int vec_ok,
                                                const u32* __restrict__ sel_ids, u64 nsel, u64 cps64, Ctrl* ctrl,
                                                u32* __restrict__ skeys, u32* __restrict__ sidx, u32* __restrict__ cnt,
                                                u32* ghist2, int hshift2, int bins2) {
     __shared__ u32 scratch[WG / WAVE];
     __shared__ u32 lh[HIST2 ? 256 : 1];
-    __shared__ u32 sk[STAGE];
-    __shared__ u32 si[IDX ? STAGE : 1];
+    __shared__ u32 sk[GSTAGE];
+    __shared__ u32 si[IDX ? GSTAGE : 1];
     h2_init<HIST2>(lh, bins2);
     const u32 thr = ctrl->first.result;
     const u32 n = (u32)n64;
@@ -378,12 +388,8 @@ __global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64
         GatherKeys ka;
         gather_load_ids(sel_ids, V, alpha, cb, ia);
         if (cb + 1 < ce) gather_load_ids(sel_ids, V, alpha, cb + 1, ib);
-        gather_load_keys(keys, n, V, alpha, vec_ok, cb, ia, ka);
+        gather_load_keys<VEC>(keys, n, V, alpha, cb, ia, ka);
         for (u32 ch = cb; ch < ce; ++ch) {
-            GatherIds ic;
-            GatherKeys kb;
-            if (ch + 2 < ce) gather_load_ids(sel_ids, V, alpha, ch + 2, ic);
-            if (ch + 1 < ce) gather_load_keys(keys, n, V, alpha, vec_ok, ch + 1, ib, kb);
             const u32 v0 = ch * (u32)TT + threadIdx.x * TE;
             u32 kv[TE];
             u32 take = 0;  // bit j: element j passes
@@ -392,6 +398,8 @@ __global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64
                 const u32 vp = v0 + 4 * q;
                 const u32 gq = (ia.v[q] << alpha) + (vp & Lm);
                 const bool gwhole = vp + 4 <= V && gq + 4 <= n;
+                if (VEC && !gwhole && vp < V && gq < n)  // the input's partial last granule
+                    for (int c = 0; c < 4; ++c) ka.v[q][c] = gq + c < n ? keys[gq + c] : 0u;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const u32 o = K::to_ord(ka.v[q][c]);
@@ -400,19 +408,34 @@ __global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64
                     if (ok && (!filter || o >= thr)) take |= 1u << (4 * q + c);
                 }
             }
+            // prefetch after this chunk's keys are consumed (a later wait would cover them too)
+            GatherIds ic;
+            GatherKeys kb;
+            if (ch + 2 < ce) gather_load_ids(sel_ids, V, alpha, ch + 2, ic);
+            if (ch + 1 < ce) gather_load_keys<VEC>(keys, n, V, alpha, ch + 1, ib, kb);
             u32 tot;
             u32 slot = block_excl_scan_u32<WG>((u32)__popc(take), scratch, &tot);
+            const u64 gpos = (u64)s * segcap + run;
+            const bool staged = tot <= GSTAGE;  // uniform; else survivors are stored directly
+            u32* __restrict__ gk = skeys + gpos;
+            u32* __restrict__ gi = IDX ? sidx + gpos : nullptr;
 #pragma unroll
             for (int j = 0; j < TE; ++j) {
                 if (take & (1u << j)) {
-                    sk[slot] = kv[j];
-                    if (IDX) si[slot] = (ia.v[j >> 2] << alpha) + ((v0 + j) & Lm);
+                    const u32 gid = (ia.v[j >> 2] << alpha) + ((v0 + j) & Lm);
+                    if (staged) {
+                        sk[slot] = kv[j];
+                        if (IDX) si[slot] = gid;
+                    } else {
+                        gk[slot] = kv[j];
+                        if (IDX) gi[slot] = gid;
+                        if (HIST2) atomicAdd(&lh[kv[j] >> hshift2], 1u);
+                    }
                     ++slot;
                 }
             }
             __syncthreads();
-            const u64 gpos = (u64)s * segcap + run;
-            flush_stage<IDX, HIST2>(sk, si, tot, skeys + gpos, IDX ? sidx + gpos : nullptr, lh, hshift2);
+            if (staged) flush_stage<IDX, HIST2>(sk, si, tot, gk, gi, lh, hshift2);
             __syncthreads();
             run += tot;
             ia = ib;
@@ -435,7 +458,10 @@ hipError_t launch_gather(const GatherArgs& a, hipStream_t st) {
                a.ghist2, a.hshift2, a.bins2
 #define G_LAUNCH(K, H, I)                                                                       \
     do {                                                                                        \
-        if (a.alpha >= 2) hipLaunchKernelGGL((k_gather<K, H, I>), dim3(grid), dim3(WG), 0, st, G_ARGS); \
+        if (a.alpha >= 2 && vec_ok && a.n >= 4)                                                 \
+            hipLaunchKernelGGL((k_gather<K, H, I, true>), dim3(grid), dim3(WG), 0, st, G_ARGS); \
+        else if (a.alpha >= 2)                                                                  \
+            hipLaunchKernelGGL((k_gather<K, H, I, false>), dim3(grid), dim3(WG), 0, st, G_ARGS); \
         else hipLaunchKernelGGL((k_gather_small<K, H, I>), dim3(grid), dim3(WG), 0, st, G_ARGS);  \
     } while (0)
     const bool h = a.ghist2 != nullptr, i = a.sidx != nullptr;
@@ -471,7 +497,9 @@ constexpr int BSPT = DRTOPK_BG_BSPT;  // subranges per thread of the beta gather
 #define DRTOPK_BG_STAGE 1024  // with 5 waves per SIMD below: 2-4% faster than 2048 at 4 waves
 #endif
 constexpr u32 BSTAGE = DRTOPK_BG_STAGE;  // staged survivors per chunk (more: direct stores)
-constexpr u32 KBUF = DRTOPK_BG_STAGE;  // keys of listed subranges held in LDS
+#ifndef DRTOPK_BG_KBUF
+#define DRTOPK_BG_KBUF 1024  // keys of listed subranges held in LDS at 5 waves per SIMD (2048, 4096: slower)
+#endif
 
 template <int BETA, bool WHOLE = false>
 __device__ __forceinline__ void load_deleg(const u32* __restrict__ dvals, u64 s0, u64 S, u32 (&d)[BSPT][BETA]) {
@@ -546,7 +574,7 @@ __device__ __forceinline__ void load_doff(const void* __restrict__ doff, u64 s0,
 // k = 2^24, alpha 4), so everything below works in 32 bits: n < 2^31 keeps key and subrange ids,
 // and chunk-local offsets, inside u32.  A chunk whose subranges are all whole (every chunk but
 // the last) skips the per-subrange and per-key bounds checks.
-template <class K, int BETA, bool POS, bool HIST2, int WAVES>
+template <class K, int BETA, bool POS, bool HIST2, int WAVES, u32 KBUF>
 __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict__ keys, u64 n64, int alpha, int filter,
                                                     const u32* __restrict__ dvals, const void* __restrict__ doff,
                                                     u64 cps64, Ctrl* ctrl, u32* __restrict__ skeys,
@@ -654,43 +682,60 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
                     if (kind[j] == 1) slist[q++] = (uint16_t)(l0 + j);
             }
             __syncthreads();
-            const bool inl = alpha >= 2 && (ns << alpha) <= KBUF;  // uniform
+            // uniform; with ns == 0 nothing is listed (slist holds no entry of this chunk)
+            const bool inl = alpha >= 2 && ns > 0 && (ns << alpha) <= KBUF;
             if (inl) {
-                const u32 ngran = (ns << alpha) >> 2;
+                const u32 ngran = (ns << alpha) >> 2;  // <= KBUF / 4 = KU * WG granules
                 const int lg = alpha - 2;
-                for (u32 x0 = 0; x0 < ngran; x0 += 2 * WG) {
-                    u32x4 r[2];
+                constexpr int KU = KBUF / (4 * WG);
+                u32x4 r[KU];
+                u32 e[KU];
 #pragma unroll
-                    for (int u = 0; u < 2; ++u) {
-                        const u32 x = x0 + u * WG + threadIdx.x;
+                for (int u = 0; u < KU; ++u) {
+                    const u32 x = u * WG + threadIdx.x;
+                    const u32 xc = x < ngran ? x : 0u;
+                    // the mask keeps the address inside this chunk's keys whatever slist holds
+                    const u32 sub = (u32)slist[xc >> lg] & (BETA_TILE_SUBS - 1);
+                    e[u] = kb + (sub << alpha) + ((xc & ((1u << lg) - 1)) << 2);
+                }
+                if (full) {
+                    // unconditional in-bounds loads (past ngran they re-read listed subrange 0):
+                    // a load under a per-lane condition is waited for where the branches join
+#pragma unroll
+                    for (int u = 0; u < KU; ++u) r[u] = *(const u32x4*)(keys + e[u]);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < KU; ++u) {
+                        const u32 x = u * WG + threadIdx.x;
                         r[u] = u32x4{0, 0, 0, 0};
                         if (x < ngran) {
-                            const u32 e = kb + ((u32)slist[x >> lg] << alpha) + ((x & ((1u << lg) - 1)) << 2);
-                            if (full || e + 4 <= n) r[u] = *(const u32x4*)(keys + e);
+                            if (e[u] + 4 <= n) r[u] = *(const u32x4*)(keys + e[u]);
                             else
                                 for (int jj = 0; jj < 4; ++jj)
-                                    if (e + jj < n) r[u][jj] = keys[e + jj];
+                                    if (e[u] + jj < n) r[u][jj] = keys[e[u] + jj];
                         }
                     }
+                }
 #pragma unroll
-                    for (int u = 0; u < 2; ++u) {
-                        const u32 x = x0 + u * WG + threadIdx.x;
-                        if (x < ngran) {
-                            u32x4 v = r[u];
-                            v.x = K::to_ord(v.x); v.y = K::to_ord(v.y); v.z = K::to_ord(v.z); v.w = K::to_ord(v.w);
-                            *(u32x4*)(kbuf + 4 * x) = v;
-                        }
+                for (int u = 0; u < KU; ++u) {
+                    const u32 x = u * WG + threadIdx.x;
+                    if (x < ngran) {
+                        u32x4 v = r[u];
+                        v.x = K::to_ord(v.x); v.y = K::to_ord(v.y); v.z = K::to_ord(v.z); v.w = K::to_ord(v.w);
+                        *(u32x4*)(kbuf + 4 * x) = v;
                     }
                 }
                 __syncthreads();
             }
             // ordered keys and kept mask of granule gl (4 keys) of listed subrange i
+            // (i < ns for the caller's use of the result; i >= ns reads listed subrange 0)
             auto fetch4 = [&](u32 i, u32* kv, u32* e_out) -> u32 {
-                const u32 e = kb + ((u32)slist[i] << alpha) + 4u * (u32)gl;
+                const u32 ic = i < ns ? i : 0u;
+                const u32 e = kb + (((u32)slist[ic] & (BETA_TILE_SUBS - 1)) << alpha) + 4u * (u32)gl;
                 *e_out = e;
                 u32 okm = 0xF;
                 if (inl) {
-                    const u32x4 v = *(const u32x4*)(kbuf + (i << alpha) + 4 * gl);
+                    const u32x4 v = *(const u32x4*)(kbuf + (ic << alpha) + 4 * gl);
                     kv[0] = v.x; kv[1] = v.y; kv[2] = v.z; kv[3] = v.w;
                     if (!full)
                         for (int jj = 0; jj < 4; ++jj)
@@ -713,11 +758,9 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
             if (alpha >= 2) {
                 for (u32 i0 = (u32)wv * per_wave; i0 < ns; i0 += (WG / WAVE) * per_wave) {  // uniform per wave
                     const u32 i = i0 + grp;
-                    u32 c = 0;
-                    if (i < ns) {
-                        u32 kv[4], e;
-                        c = (u32)__popc(fetch4(i, kv, &e));
-                    }
+                    u32 kv[4], e;
+                    const u32 okm = fetch4(i, kv, &e);  // ns >= 1 here, so listed subrange 0 exists
+                    u32 c = i < ns ? (u32)__popc(okm) : 0u;
                     for (int o = 1; o < G; o <<= 1) c += (u32)__shfl_xor((int)c, o, WAVE);
                     if (gl == 0 && i < ns) spos[i] = c;
                 }
@@ -814,8 +857,9 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
             if (alpha >= 2) {
                 for (u32 i0 = (u32)wv * per_wave; i0 < ns; i0 += (WG / WAVE) * per_wave) {
                     const u32 i = i0 + grp;
-                    u32 kv[4] = {0, 0, 0, 0}, okm = 0, e = 0;
-                    if (i < ns) okm = fetch4(i, kv, &e);
+                    u32 kv[4], e;
+                    u32 okm = fetch4(i, kv, &e);
+                    if (i >= ns) okm = 0;
                     const u32 c4 = (u32)__popc(okm);
                     u32 incl = c4;
                     for (int o = 1; o < G; o <<= 1) {
@@ -889,15 +933,15 @@ static hipError_t launch_beta_t(const BetaGatherArgs& a, hipStream_t st) {
     const bool pos = a.sidx != nullptr, h = a.ghist2 != nullptr;
     // 32-key subranges (alpha 5) run faster at 8 waves per SIMD (64 VGPRs); at alpha <= 4 the
     // random 16-byte subrange reads are the limit and 5 waves do better
-#define B_LAUNCH(W)                                                                                              \
+#define B_LAUNCH(W, KB)                                                                                          \
     do {                                                                                                         \
-        if (pos && h) hipLaunchKernelGGL((k_beta_gather<K, BETA, true, true, W>), dim3(grid), dim3(WG), 0, st, B_ARGS);   \
-        else if (pos) hipLaunchKernelGGL((k_beta_gather<K, BETA, true, false, W>), dim3(grid), dim3(WG), 0, st, B_ARGS);  \
-        else if (h) hipLaunchKernelGGL((k_beta_gather<K, BETA, false, true, W>), dim3(grid), dim3(WG), 0, st, B_ARGS);    \
-        else hipLaunchKernelGGL((k_beta_gather<K, BETA, false, false, W>), dim3(grid), dim3(WG), 0, st, B_ARGS);          \
+        if (pos && h) hipLaunchKernelGGL((k_beta_gather<K, BETA, true, true, W, KB>), dim3(grid), dim3(WG), 0, st, B_ARGS);   \
+        else if (pos) hipLaunchKernelGGL((k_beta_gather<K, BETA, true, false, W, KB>), dim3(grid), dim3(WG), 0, st, B_ARGS);  \
+        else if (h) hipLaunchKernelGGL((k_beta_gather<K, BETA, false, true, W, KB>), dim3(grid), dim3(WG), 0, st, B_ARGS);    \
+        else hipLaunchKernelGGL((k_beta_gather<K, BETA, false, false, W, KB>), dim3(grid), dim3(WG), 0, st, B_ARGS);          \
     } while (0)
-    if (a.alpha >= 5) B_LAUNCH(8);
-    else B_LAUNCH(5);
+    if (a.alpha >= 5) B_LAUNCH(8, 1024);
+    else B_LAUNCH(5, DRTOPK_BG_KBUF);
 #undef B_LAUNCH
 #undef B_ARGS
     return hipGetLastError();
