@@ -338,6 +338,14 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
         sa.fuse = radix ? (fuse_first ? 1 : 0) : 2;
         sa.dvals = dvals; sa.dpos = doff; sa.ghist = hist;
         sa.hshift = 32 - nb; sa.bins = bins; sa.ctrl = ctrl;
+        // beta path, radix, 8-bit digits, beta 2, FILTER, fused first digit: the threshold is a
+        // top-digit boundary and no pass reads the delegates but the concatenation, which needs
+        // the second delegate only against it (lonely delegates below it are dropped), so the
+        // delegates are stored as the first one plus the second's top byte (5 bytes, not 8)
+        const bool top_form = pl.path == 2 && radix && nb == 8 && pl.beta == 2 &&
+                              (pl.flags & DRTOPK_FLAG_FILTER) && fuse_first && sa.staged;
+        u8* dtop = top_form ? (u8*)(dvals + pl.S) : nullptr;  // inside the 8*S-byte delegate area
+        sa.dtop = dtop;
         HIPCHK(launch_sample(sa, st));
         HIPCHK(mark(1));
         // ---- first top-k over the delegates
@@ -375,7 +383,7 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
             // threshold are counted, not written
             const int bfilter = (pl.flags & DRTOPK_FLAG_FILTER) ? (radix ? 3 : 1) : 0;
             BetaGatherArgs ba{keys, pl.n, pl.alpha, pl.beta, pl.f32, bfilter,
-                              dvals, doff, ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins};
+                              dvals, doff, ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins, dtop};
             HIPCHK(launch_beta_gather(ba, st));
         }
         HIPCHK(launch_seg_total(scnt, pl.nseg, ctrl, st));

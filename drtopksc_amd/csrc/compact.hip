@@ -579,7 +579,7 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
                                                     const u32* __restrict__ dvals, const void* __restrict__ doff,
                                                     u64 cps64, Ctrl* ctrl, u32* __restrict__ skeys,
                                                     u32* __restrict__ sidx, u32* __restrict__ cnt, u32* ghist2,
-                                                    int hshift2, int bins2) {
+                                                    int hshift2, int bins2, const u8* __restrict__ dtop) {
     __shared__ u32 scratch[WG / WAVE];
     __shared__ u32 lh[HIST2 ? 256 : 1];
     __shared__ u64 red[4][WG / WAVE];
@@ -629,7 +629,32 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
                 if (POS) load_doff<BETA>(doff, c0 + BETA_TILE_SUBS + l0, S, pkn);
             }
 #else
-            if (full) {
+            if (BETA == 2 && dtop) {
+                static_assert(BSPT == 4, "the top-byte delegate form is read as 4 subranges per thread");
+                // first delegates and the second's top bytes: the threshold's low 24 bits are 0,
+                // so (top << 24) >= threshold exactly when the second delegate is
+                u32 m1[BSPT], tops;
+                if (full) {
+                    const u32x4 q = *(const u32x4*)(dvals + c0 + l0);
+                    m1[0] = q.x; m1[1] = q.y; m1[2] = q.z; m1[3] = q.w;
+                    tops = *(const u32*)(dtop + c0 + l0);
+                    if (POS) load_doff<BETA, true>(doff, c0 + l0, S, pk);
+                } else {
+                    tops = 0;
+#pragma unroll
+                    for (int j = 0; j < BSPT; ++j) {
+                        const u32 sj = c0 + l0 + j;
+                        m1[j] = sj < S ? dvals[sj] : 0u;
+                        tops |= (sj < S ? (u32)dtop[sj] : 0u) << (8 * j);
+                    }
+                    if (POS) load_doff<BETA>(doff, c0 + l0, S, pk);
+                }
+#pragma unroll
+                for (int j = 0; j < BSPT; ++j) {
+                    d[j][0] = m1[j];
+                    d[j][BETA - 1] = ((tops >> (8 * j)) & 0xFFu) << 24;
+                }
+            } else if (full) {
                 load_deleg<BETA, true>(dvals, c0 + l0, S, d);
                 if (POS) load_doff<BETA, true>(doff, c0 + l0, S, pk);
             } else {
@@ -929,7 +954,7 @@ static hipError_t launch_beta_t(const BetaGatherArgs& a, hipStream_t st) {
     const u64 nseg = (nch + cps - 1) / cps;
     const unsigned grid = (unsigned)std::min<u64>(nseg, CGRID);
 #define B_ARGS a.keys, a.n, a.alpha, a.filter, a.dvals, a.doff, cps, a.ctrl, a.skeys, a.sidx, a.cnt, a.ghist2, \
-               a.hshift2, a.bins2
+               a.hshift2, a.bins2, BETA == 2 ? a.dtop : nullptr
     const bool pos = a.sidx != nullptr, h = a.ghist2 != nullptr;
     // 32-key subranges (alpha 5) run faster at 8 waves per SIMD (64 VGPRs); at alpha <= 4 the
     // random 16-byte subrange reads are the limit and 5 waves do better
@@ -956,6 +981,9 @@ static hipError_t launch_beta_k(const BetaGatherArgs& a, hipStream_t st) {
     return hipErrorInvalidValue;
 }
 hipError_t launch_beta_gather(const BetaGatherArgs& a, hipStream_t st) {
+    // the top-byte form holds only what a threshold on the top 8-bit digit with lonely
+    // delegates below it dropped can use
+    if (a.dtop && !(a.beta == 2 && (a.filter & 2))) return hipErrorInvalidValue;
     return a.f32 ? launch_beta_k<KeyF32>(a, st) : launch_beta_k<KeyU32>(a, st);
 }
 

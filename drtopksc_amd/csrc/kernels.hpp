@@ -29,6 +29,9 @@ struct SampleArgs {
     u32* ghist;        // fused histogram (fuse == 1)
     int hshift, bins;  // fused histogram: bin = delegate >> hshift, bins bins
     Ctrl* ctrl;
+    // beta 2, staged: when set, dvals[s] holds the first delegate and dtop[s] the top byte of the
+    // second (all a beta-path threshold on the top 8-bit digit needs of it)
+    u8* dtop = nullptr;
 };
 hipError_t launch_sample(const SampleArgs& a, hipStream_t st);
 
@@ -104,6 +107,7 @@ struct BetaGatherArgs {
     u32* cnt;
     u32* ghist2;
     int hshift2, bins2;
+    const u8* dtop = nullptr;  // beta 2: delegates as the sampler's dtop form (dvals[s] first, dtop[s] top byte of the second)
 };
 #ifndef DRTOPK_BG_BSPT
 #define DRTOPK_BG_BSPT 4  // 4 measured faster than 8 (fewer VGPRs, more resident workgroups)

@@ -132,12 +132,23 @@ struct ListOf<2, 2> {
 // POSM: 0 none, 1 absolute positions (u32 per delegate), 2 packed offsets of the first
 // beta-1 delegates inside the subrange (8 bits each; u8 / u16 / u32 per subrange).
 template <int BETA, int POSM, class LT>
-__device__ __forceinline__ void store_delegates(const LT& t, u64 s, int alpha, u32* dvals, void* dpos) {
+__device__ __forceinline__ void store_delegates(const LT& t, u64 s, int alpha, u32* dvals, void* dpos, u8* dtop) {
     if constexpr (std::is_same<LT, Top2P>::value) {
-        // both delegates with one 8-byte store, the first maximum's offset as one byte
-        *(uint2*)(dvals + 2 * s) = make_uint2(t.m1, t.m2);
+        // both delegates with one 8-byte store (or the first and the second's top byte), the
+        // first maximum's offset as one byte
+        if (dtop) {
+            dvals[s] = t.m1;
+            dtop[s] = (u8)(t.m2 >> 24);
+        } else {
+            *(uint2*)(dvals + 2 * s) = make_uint2(t.m1, t.m2);
+        }
         const u32 off = t.p1 == 0xFFFFFFFFu ? 0xFFu : (u32)(t.p1 - (u32)(s << alpha)) & 0xFFu;
         ((uint8_t*)dpos)[s] = (uint8_t)off;
+        return;
+    }
+    if (BETA == 2 && POSM == 0 && dtop) {
+        dvals[s] = t.val(0);
+        dtop[s] = (u8)(t.val(1) >> 24);
         return;
     }
     u32 packed = 0;
@@ -251,7 +262,7 @@ __global__ __launch_bounds__(WG) void k_sample_groups(const u32* __restrict__ ke
                     for (int c = 0; c < 2; ++c)
                         if (i0 + 2 * h + c < n) t.insert(kv[2 * h + c], i0 + 2 * h + c);
                     const bool leader = i0 + 2 * h < n;
-                    if (leader) store_delegates<BETA, POSM, LT>(t, (i0 >> 1) + h, alpha, dvals, dpos);
+                    if (leader) store_delegates<BETA, POSM, LT>(t, (i0 >> 1) + h, alpha, dvals, dpos, nullptr);
                     fuse_add<BETA, FUSE, LT>(t, leader, lhist, hshift, &mn, &mx);
                 }
             } else {
@@ -269,7 +280,7 @@ __global__ __launch_bounds__(WG) void k_sample_groups(const u32* __restrict__ ke
                 for (int off = g >> 1; off > 0; off >>= 1) t.shfl_merge(off);
                 const u64 s = i0 >> alpha;
                 const bool leader = (lane & (g - 1)) == 0 && i0 < n && s < S;
-                if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos);
+                if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos, nullptr);
                 fuse_add<BETA, FUSE, LT>(t, leader, lhist, hshift, &mn, &mx);
             }
         }
@@ -317,7 +328,7 @@ __global__ __launch_bounds__(WG) void k_sample_waves(const u32* __restrict__ key
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) t.shfl_merge(off);
                 const u64 s = (u0 >> alpha) + r;
-                if (lane == 0) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos);
+                if (lane == 0) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos, nullptr);
                 fuse_add<BETA, FUSE, LT>(t, lane == 0, lhist, hshift, &mn, &mx);
             }
         } else if (full) {
@@ -347,7 +358,7 @@ __global__ __launch_bounds__(WG) void k_sample_waves(const u32* __restrict__ key
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) t.shfl_merge(off);
             const u64 s = u0 >> alpha;
-            if (lane == 0) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos);
+            if (lane == 0) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos, nullptr);
             fuse_add<BETA, FUSE, LT>(t, lane == 0, lhist, hshift, &mn, &mx);
         } else {
             // tail unit: scalar loads, one subrange at a time
@@ -361,7 +372,7 @@ __global__ __launch_bounds__(WG) void k_sample_waves(const u32* __restrict__ key
                 for (int off = 32; off > 0; off >>= 1) t.shfl_merge(off);
                 const u64 s = sb >> alpha;
                 const bool leader = lane == 0 && s < S;
-                if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos);
+                if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos, nullptr);
                 fuse_add<BETA, FUSE, LT>(t, leader, lhist, hshift, &mn, &mx);
             }
         }
@@ -433,8 +444,8 @@ __device__ __forceinline__ void staged_reduce_full(const u32x4* tile, int j, u64
 }
 
 template <int A, int BETA, int POSM, int FUSE, class LT>
-__device__ __forceinline__ void staged_tile_full(const u32x4* tile, u64 k0, u32* dvals, void* dpos, u32* lhist,
-                                                 int hshift, u32* mn, u32* mx) {
+__device__ __forceinline__ void staged_tile_full(const u32x4* tile, u64 k0, u32* dvals, void* dpos, u8* dtop,
+                                                 u32* lhist, int hshift, u32* mn, u32* mx) {
     constexpr int SPT = STAGE_KEYS >> A;
 #pragma unroll
     for (int j0 = 0; j0 < SPT; j0 += WG) {
@@ -443,7 +454,7 @@ __device__ __forceinline__ void staged_tile_full(const u32x4* tile, u64 k0, u32*
         LT t;
         t.clear();
         staged_reduce_full<A, LT>(tile, j, k0 + ((u64)j << A), t);
-        store_delegates<BETA, POSM, LT>(t, s, A, dvals, dpos);
+        store_delegates<BETA, POSM, LT>(t, s, A, dvals, dpos, dtop);
         fuse_add<BETA, FUSE, LT>(t, true, lhist, hshift, mn, mx);
     }
 }
@@ -451,7 +462,8 @@ __device__ __forceinline__ void staged_tile_full(const u32x4* tile, u64 k0, u32*
 template <class K, int BETA, int POSM, int FUSE, class LT>
 __global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ keys, u64 n, int alpha, int vec_ok,
                                                       u64 ntiles, u32* __restrict__ dvals, void* __restrict__ dpos,
-                                                      u32* __restrict__ ghist, int hshift, int bins, Ctrl* ctrl) {
+                                                      u32* __restrict__ ghist, int hshift, int bins, Ctrl* ctrl,
+                                                      u8* __restrict__ dtop) {
     // one pad uint4 after every 8 uint4 spreads the threads' subrange reads over the LDS banks
     __shared__ u32x4 tile[STAGE_Q * WG + STAGE_Q * WG / 8];
     __shared__ u32 lhist[FUSE == 1 ? 256 : 1];
@@ -486,10 +498,10 @@ __global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ ke
         const u64 s0 = k0 >> alpha;
         if (vec_ok && k0 + STAGE_KEYS <= n && alpha >= 2) {  // uniform: a full tile
             switch (alpha) {
-                case 2: staged_tile_full<2, BETA, POSM, FUSE, LT>(tile, k0, dvals, dpos, lhist, hshift, &mn, &mx); break;
-                case 3: staged_tile_full<3, BETA, POSM, FUSE, LT>(tile, k0, dvals, dpos, lhist, hshift, &mn, &mx); break;
-                case 4: staged_tile_full<4, BETA, POSM, FUSE, LT>(tile, k0, dvals, dpos, lhist, hshift, &mn, &mx); break;
-                default: staged_tile_full<5, BETA, POSM, FUSE, LT>(tile, k0, dvals, dpos, lhist, hshift, &mn, &mx); break;
+                case 2: staged_tile_full<2, BETA, POSM, FUSE, LT>(tile, k0, dvals, dpos, dtop, lhist, hshift, &mn, &mx); break;
+                case 3: staged_tile_full<3, BETA, POSM, FUSE, LT>(tile, k0, dvals, dpos, dtop, lhist, hshift, &mn, &mx); break;
+                case 4: staged_tile_full<4, BETA, POSM, FUSE, LT>(tile, k0, dvals, dpos, dtop, lhist, hshift, &mn, &mx); break;
+                default: staged_tile_full<5, BETA, POSM, FUSE, LT>(tile, k0, dvals, dpos, dtop, lhist, hshift, &mn, &mx); break;
             }
             __syncthreads();
             continue;
@@ -522,7 +534,7 @@ __global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ ke
                 if (kb + 1 < n) t.insert(b, kb + 1);
             }
             const bool leader = s < S;
-            if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos);
+            if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos, dtop);
             fuse_add<BETA, FUSE, LT>(t, leader, lhist, hshift, &mn, &mx);
         }
         __syncthreads();
@@ -544,7 +556,8 @@ static hipError_t launch_sample_t(const SampleArgs& a, hipStream_t st) {
         const u64 ntiles = (a.n + STAGE_KEYS - 1) / STAGE_KEYS;
         const unsigned grid = (unsigned)std::min<u64>(ntiles, 4ull * device_cus());
         hipLaunchKernelGGL((k_sample_staged<K, BETA, POSM, FUSE, LT>), dim3(grid), dim3(WG), 0, st, a.keys, a.n,
-                           a.alpha, vec_ok, ntiles, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl);
+                           a.alpha, vec_ok, ntiles, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl,
+                           BETA == 2 ? a.dtop : nullptr);
     } else if (a.alpha <= 8) {
         const u64 nchunks = (a.n + GROUP_CHUNK - 1) / GROUP_CHUNK;
         const unsigned grid = (unsigned)std::min<u64>(nchunks, SAMPLE_GRID);
@@ -589,6 +602,7 @@ static hipError_t launch_sample_k(const SampleArgs& a, hipStream_t st) {
 
 hipError_t launch_sample(const SampleArgs& a, hipStream_t st) {
     if (a.posmode == 2 && a.beta < 2) return hipErrorInvalidValue;
+    if (a.dtop && !(a.beta == 2 && a.staged && a.alpha <= 5)) return hipErrorInvalidValue;
     return a.f32 ? launch_sample_k<KeyF32>(a, st) : launch_sample_k<KeyU32>(a, st);
 }
 
