@@ -251,11 +251,7 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
     const int nb = pl.nb, B = pl.B;
     const int D0 = 32 / nb - 1;
     const int bins = radix ? (1 << nb) : B;
-#ifdef DRTOPK_NOFUSE_BETA
-    const bool fuse_hist = radix && nb <= 8 && pl.path != 2;
-#else
     const bool fuse_hist = radix && nb <= 8;
-#endif
     // The first digit's histogram rides on the delegate pass, except for alpha 6..8: there the
     // lane-group sampler produces one delegate per 16..64 lanes and the per-wave histogram update
     // costs it more (25-60 us at 2^30 keys) than a separate pass over the delegates (11-17 us).

@@ -567,9 +567,6 @@ __device__ __forceinline__ void load_doff(const void* __restrict__ doff, u64 s0,
     }
 }
 
-#ifndef DRTOPK_BG_PF
-#define DRTOPK_BG_PF 0
-#endif
 // The kernel is bound by instruction issue rather than by HBM (about 1.3 GB moved per launch at
 // k = 2^24, alpha 4), so everything below works in 32 bits: n < 2^31 keeps key and subrange ids,
 // and chunk-local offsets, inside u32.  A chunk whose subranges are all whole (every chunk but
@@ -602,9 +599,6 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
     const int gl = lane & (G - 1), grp = lane / G, per_wave = WAVE / G;
     const u32 l0 = threadIdx.x * BSPT;  // chunk-local id of the thread's first subrange
     u32 nsel = 0, nlon = 0, nlel = 0, ndrop = 0;
-#if DRTOPK_BG_PF
-    u32 dn[BSPT][BETA], pkn[BSPT];
-#endif
     for (u32 sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
         u32 run = 0;
         const u32 ch_end = min((sg + 1) * cps, nchunks);
@@ -613,22 +607,6 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
             const u32 kb = c0 << alpha;          // its first key
             const bool full = ((u64)(c0 + BETA_TILE_SUBS) << alpha) <= n64;  // uniform
             u32 d[BSPT][BETA], pk[BSPT];
-#if DRTOPK_BG_PF
-            if (ch == sg * cps) {
-                load_deleg<BETA>(dvals, c0 + l0, S, dn);
-                if (POS) load_doff<BETA>(doff, c0 + l0, S, pkn);
-            }
-#pragma unroll
-            for (int j = 0; j < BSPT; ++j) {
-                pk[j] = pkn[j];
-#pragma unroll
-                for (int b = 0; b < BETA; ++b) d[j][b] = dn[j][b];
-            }
-            if (ch + 1 < ch_end) {  // the next chunk's delegates, in flight during this one
-                load_deleg<BETA>(dvals, c0 + BETA_TILE_SUBS + l0, S, dn);
-                if (POS) load_doff<BETA>(doff, c0 + BETA_TILE_SUBS + l0, S, pkn);
-            }
-#else
             if (BETA == 2 && dtop) {
                 static_assert(BSPT == 4, "the top-byte delegate form is read as 4 subranges per thread");
                 // first delegates and the second's top bytes: the threshold's low 24 bits are 0,
@@ -661,7 +639,6 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
                 load_deleg<BETA>(dvals, c0 + l0, S, d);
                 if (POS) load_doff<BETA>(doff, c0 + l0, S, pk);
             }
-#endif
             int kind[BSPT];  // 0 none, 1 selected (its keys), 2 lonely (its delegates)
             u32 mj[BSPT];    // lonely: delegates written (with filter bit 2 only those >= threshold)
             u32 ns_t = 0;
