@@ -116,6 +116,37 @@ def test_pipeline_default_config(orc, dt, algo):
             compare(orc, dt, name, ok, dev, k, algo, orc.FLAGS_DEFAULT, f32=f32)
 
 
+@pytest.mark.parametrize("want", ["kth", "keys", "idx"])
+def test_output_subsets(orc, dt, want):
+    """k-th key only, top-k keys only, indices only: the delegate pass keeps no positions when
+    neither output is asked for, and the beta path's compact delegate form (beta 2, FILTER,
+    8-bit digits) then takes its position-free store; the k-th key and statistics must not move."""
+    for name, v, f32 in [("uniform", orc.generate(0, 21, 1 << 20), False),
+                         ("ties8", np.random.default_rng(5).integers(0, 8, 300001, dtype=np.uint64).astype(np.uint32), False),
+                         ("f32normal", orc.generate(3, 22, (1 << 20) + 3), True)]:
+        ok = orc.f32_to_ordered(v) if f32 else v
+        dev = to_dev_f32(v) if f32 else to_dev(v)
+        for k in [1000, 5000, 20000, 70000]:
+            if k > v.size:
+                continue
+            print(f"[case] outputs={want} {name} k={k}", flush=True)
+            po = orc.params(flags=orc.FLAGS_DEFAULT)
+            K, keys, idx, ost = orc.drtopk(0, ok, k, po)
+            kth, tk, ti, gst = dt.select(dev, k, p=dt.params(), want_keys=(want == "keys"),
+                                         want_idx=(want == "idx"), stats=True)
+            gk = kth.cpu().numpy().view(np.uint32)[:1]
+            if f32:
+                gk = orc.f32_to_ordered(gk)
+            assert int(gk[0]) == K, (name, k, want, "kth")
+            if want == "keys":
+                kb = tk.cpu().numpy().view(np.uint32)
+                assert np.array_equal(orc.f32_to_ordered(kb) if f32 else kb, keys), (name, k, "keys")
+            if want == "idx":
+                assert np.array_equal(ti.cpu().numpy(), idx), (name, k, "idx")
+            for s in STAT_KEYS:
+                assert gst[s] == ost[s], (name, k, want, s, gst[s], ost[s])
+
+
 @pytest.mark.parametrize("flags", [0, 1, 2, 3, 5, 6, 7])
 @pytest.mark.parametrize("nb", [4, 8, 16])
 def test_pipeline_radix_options(orc, dt, flags, nb):
