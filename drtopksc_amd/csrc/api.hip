@@ -136,7 +136,7 @@ drtopk_status resolve(drtopk_algo algo, drtopk_dtype dtype, u64 n, u64 k, const 
         pl->nseg = gather_nseg(k * pl->L);
     } else if (path == 2) {
         pl->segcap = beta_segcap(pl->S, alpha);
-        pl->nseg = beta_nseg(pl->S);
+        pl->nseg = beta_nseg(pl->S, alpha);
     }
     pl->ext_nseg = bit ? 0 : (path == 0 ? cdiv(n, DIRECT_TILE) : pl->nseg);
     pl->range_nt = (path == 1 && !bit) ? range_tiles(pl->S) : 0;

@@ -497,6 +497,9 @@ constexpr int BSPT = DRTOPK_BG_BSPT;  // subranges per thread of the beta gather
 #define DRTOPK_BG_STAGE 1024  // with 5 waves per SIMD below: 2-4% faster than 2048 at 4 waves
 #endif
 constexpr u32 BSTAGE = DRTOPK_BG_STAGE;  // staged survivors per chunk (more: direct stores)
+#ifndef DRTOPK_BG_ONEROUND
+#define DRTOPK_BG_ONEROUND 1
+#endif
 #ifndef DRTOPK_BG_KBUF
 #define DRTOPK_BG_KBUF 1024  // keys of listed subranges held in LDS at 5 waves per SIMD (2048, 4096: slower)
 #endif
@@ -913,13 +916,31 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
     h2_flush<HIST2>(lh, ghist2, bins2);
 }
 
-u64 beta_nseg(u64 S) {
-    const u64 nch = (S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS;
+// Waves per SIMD of the beta gather: 32-key subranges (alpha 5) run faster at 8 (64 VGPRs); at
+// alpha <= 4 the random 16-byte subrange reads are the limit and 5 do better.
+static int beta_waves(int alpha) { return alpha >= 5 ? 8 : 5; }
+
+// Chunks per beta-gather segment.  A workgroup runs one segment at a time, so equal segments
+// finish in rounds of the resident workgroups (waves per SIMD x CUs); once segments would
+// exceed 16 chunks' worth of rounds, one segment per resident workgroup avoids a partly
+// filled last round (2^30 keys, alpha 4: 4096 segments = 3.2 rounds of 1280).
+static u64 beta_cps(u64 nch, int alpha) {
     const u64 cps = chunks_per_segment(nch);
+#if DRTOPK_BG_ONEROUND
+    const u64 slots = (u64)device_cus() * (u64)beta_waves(alpha);
+    if (nch > slots * cps) return (nch + slots - 1) / slots;
+#else
+    (void)alpha;
+#endif
+    return cps;
+}
+u64 beta_nseg(u64 S, int alpha) {
+    const u64 nch = (S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS;
+    const u64 cps = beta_cps(nch, alpha);
     return (nch + cps - 1) / cps;
 }
 u64 beta_segcap(u64 S, int alpha) {
-    return chunks_per_segment((S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS) * (BETA_TILE_SUBS << alpha);
+    return beta_cps((S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS, alpha) * (BETA_TILE_SUBS << alpha);
 }
 
 template <class K, int BETA>
@@ -927,14 +948,12 @@ static hipError_t launch_beta_t(const BetaGatherArgs& a, hipStream_t st) {
     const u64 L = 1ull << a.alpha;
     const u64 S = (a.n + L - 1) >> a.alpha;
     const u64 nch = (S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS;
-    const u64 cps = chunks_per_segment(nch);
+    const u64 cps = beta_cps(nch, a.alpha);
     const u64 nseg = (nch + cps - 1) / cps;
     const unsigned grid = (unsigned)std::min<u64>(nseg, CGRID);
 #define B_ARGS a.keys, a.n, a.alpha, a.filter, a.dvals, a.doff, cps, a.ctrl, a.skeys, a.sidx, a.cnt, a.ghist2, \
                a.hshift2, a.bins2, BETA == 2 ? a.dtop : nullptr
     const bool pos = a.sidx != nullptr, h = a.ghist2 != nullptr;
-    // 32-key subranges (alpha 5) run faster at 8 waves per SIMD (64 VGPRs); at alpha <= 4 the
-    // random 16-byte subrange reads are the limit and 5 waves do better
 #define B_LAUNCH(W, KB)                                                                                          \
     do {                                                                                                         \
         if (pos && h) hipLaunchKernelGGL((k_beta_gather<K, BETA, true, true, W, KB>), dim3(grid), dim3(WG), 0, st, B_ARGS);   \
@@ -942,7 +961,7 @@ static hipError_t launch_beta_t(const BetaGatherArgs& a, hipStream_t st) {
         else if (h) hipLaunchKernelGGL((k_beta_gather<K, BETA, false, true, W, KB>), dim3(grid), dim3(WG), 0, st, B_ARGS);    \
         else hipLaunchKernelGGL((k_beta_gather<K, BETA, false, false, W, KB>), dim3(grid), dim3(WG), 0, st, B_ARGS);          \
     } while (0)
-    if (a.alpha >= 5) B_LAUNCH(8, 1024);
+    if (beta_waves(a.alpha) == 8) B_LAUNCH(8, 1024);
     else B_LAUNCH(5, DRTOPK_BG_KBUF);
 #undef B_LAUNCH
 #undef B_ARGS

@@ -115,7 +115,7 @@ struct BetaGatherArgs {
 constexpr u64 BETA_TILE_SUBS = 256 * DRTOPK_BG_BSPT;  // subranges per chunk of the beta gather
 hipError_t launch_beta_gather(const BetaGatherArgs& a, hipStream_t st);
 u64 beta_segcap(u64 S, int alpha);
-u64 beta_nseg(u64 S);
+u64 beta_nseg(u64 S, int alpha);
 
 // concatenation size = sum of segment counts -> ctrl->concat_size, second.m
 hipError_t launch_seg_total(const u32* cnt, u64 nseg, Ctrl* ctrl, hipStream_t st);
