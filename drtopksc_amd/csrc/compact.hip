@@ -602,6 +602,11 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
     const int gl = lane & (G - 1), grp = lane / G, per_wave = WAVE / G;
     const u32 l0 = threadIdx.x * BSPT;  // chunk-local id of the thread's first subrange
     u32 nsel = 0, nlon = 0, nlel = 0, ndrop = 0;
+    // 5 waves per SIMD leave room for the compact delegates of the next chunk (6 VGPRs): they
+    // are loaded while this chunk is processed (at 8 waves they would spill)
+    constexpr bool PF = WAVES <= 6;
+    u32x4 pf_q = {0, 0, 0, 0};
+    u32 pf_t = 0, pf_p = 0;
     for (u32 sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
         u32 run = 0;
         const u32 ch_end = min((sg + 1) * cps, nchunks);
@@ -616,10 +621,34 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
                 // so (top << 24) >= threshold exactly when the second delegate is
                 u32 m1[BSPT], tops;
                 if (full) {
-                    const u32x4 q = *(const u32x4*)(dvals + c0 + l0);
-                    m1[0] = q.x; m1[1] = q.y; m1[2] = q.z; m1[3] = q.w;
-                    tops = *(const u32*)(dtop + c0 + l0);
-                    if (POS) load_doff<BETA, true>(doff, c0 + l0, S, pk);
+                    if constexpr (PF) {
+                        // this chunk's words were prefetched during the previous chunk of the
+                        // segment (from a clamped, whole chunk: always in bounds)
+                        u32x4 q;
+                        u32 pkw;
+                        if (ch == sg * cps) {
+                            q = *(const u32x4*)(dvals + c0 + l0);
+                            tops = *(const u32*)(dtop + c0 + l0);
+                            pkw = POS ? *(const u32*)((const u8*)doff + c0 + l0) : 0u;
+                        } else {
+                            q = pf_q;
+                            tops = pf_t;
+                            pkw = pf_p;
+                        }
+                        const u32 cn = (ch + 1 < ch_end && ((u64)(c0 + 2 * BETA_TILE_SUBS) << alpha) <= n64)
+                                           ? c0 + BETA_TILE_SUBS : c0;
+                        pf_q = *(const u32x4*)(dvals + cn + l0);
+                        pf_t = *(const u32*)(dtop + cn + l0);
+                        pf_p = POS ? *(const u32*)((const u8*)doff + cn + l0) : 0u;
+                        m1[0] = q.x; m1[1] = q.y; m1[2] = q.z; m1[3] = q.w;
+                        if (POS)
+                            for (int j = 0; j < BSPT; ++j) pk[j] = (pkw >> (8 * j)) & 0xFFu;
+                    } else {
+                        const u32x4 q = *(const u32x4*)(dvals + c0 + l0);
+                        m1[0] = q.x; m1[1] = q.y; m1[2] = q.z; m1[3] = q.w;
+                        tops = *(const u32*)(dtop + c0 + l0);
+                        if (POS) load_doff<BETA, true>(doff, c0 + l0, S, pk);
+                    }
                 } else {
                     tops = 0;
 #pragma unroll
