@@ -337,10 +337,12 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
         // beta path, radix, 8-bit digits, beta 2, FILTER, fused first digit: the threshold is a
         // top-digit boundary and no pass reads the delegates but the concatenation, which needs
         // the second delegate only against it (lonely delegates below it are dropped), so the
-        // delegates are stored as the first one plus the second's top byte (5 bytes, not 8)
+        // delegates are stored as the first one plus one 16-bit word holding the second's top
+        // byte and the first's offset (6 bytes per subrange, not 8 + 1)
         const bool top_form = pl.path == 2 && radix && nb == 8 && pl.beta == 2 &&
                               (pl.flags & DRTOPK_FLAG_FILTER) && fuse_first && sa.staged;
-        u8* dtop = top_form ? (u8*)(dvals + pl.S) : nullptr;  // inside the 8*S-byte delegate area
+        // 16-bit words after the first delegates, 16-byte aligned, inside the 8*S-byte area
+        u8* dtop = top_form ? (u8*)(dvals + align_up(pl.S, 4)) : nullptr;
         sa.dtop = dtop;
         HIPCHK(launch_sample(sa, st));
         HIPCHK(mark(1));

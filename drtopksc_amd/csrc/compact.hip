@@ -606,7 +606,7 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
     // are loaded while this chunk is processed (at 8 waves they would spill)
     constexpr bool PF = WAVES <= 6;
     u32x4 pf_q = {0, 0, 0, 0};
-    u32 pf_t = 0, pf_p = 0;
+    u64 pf_t = 0;
     for (u32 sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
         u32 run = 0;
         const u32 ch_end = min((sg + 1) * cps, nchunks);
@@ -619,50 +619,45 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
                 static_assert(BSPT == 4, "the top-byte delegate form is read as 4 subranges per thread");
                 // first delegates and the second's top bytes: the threshold's low 24 bits are 0,
                 // so (top << 24) >= threshold exactly when the second delegate is
-                u32 m1[BSPT], tops;
+                u32 m1[BSPT];
+                u64 tw;  // per subrange: the second's top byte, then the first's offset byte
                 if (full) {
                     if constexpr (PF) {
                         // this chunk's words were prefetched during the previous chunk of the
                         // segment (from a clamped, whole chunk: always in bounds)
                         u32x4 q;
-                        u32 pkw;
                         if (ch == sg * cps) {
                             q = *(const u32x4*)(dvals + c0 + l0);
-                            tops = *(const u32*)(dtop + c0 + l0);
-                            pkw = POS ? *(const u32*)((const u8*)doff + c0 + l0) : 0u;
+                            tw = *(const u64*)(dtop + 2 * (c0 + l0));
                         } else {
                             q = pf_q;
-                            tops = pf_t;
-                            pkw = pf_p;
+                            tw = pf_t;
                         }
                         const u32 cn = (ch + 1 < ch_end && ((u64)(c0 + 2 * BETA_TILE_SUBS) << alpha) <= n64)
                                            ? c0 + BETA_TILE_SUBS : c0;
                         pf_q = *(const u32x4*)(dvals + cn + l0);
-                        pf_t = *(const u32*)(dtop + cn + l0);
-                        pf_p = POS ? *(const u32*)((const u8*)doff + cn + l0) : 0u;
+                        pf_t = *(const u64*)(dtop + 2 * (cn + l0));
                         m1[0] = q.x; m1[1] = q.y; m1[2] = q.z; m1[3] = q.w;
-                        if (POS)
-                            for (int j = 0; j < BSPT; ++j) pk[j] = (pkw >> (8 * j)) & 0xFFu;
                     } else {
                         const u32x4 q = *(const u32x4*)(dvals + c0 + l0);
                         m1[0] = q.x; m1[1] = q.y; m1[2] = q.z; m1[3] = q.w;
-                        tops = *(const u32*)(dtop + c0 + l0);
-                        if (POS) load_doff<BETA, true>(doff, c0 + l0, S, pk);
+                        tw = *(const u64*)(dtop + 2 * (c0 + l0));
                     }
                 } else {
-                    tops = 0;
+                    tw = 0;
 #pragma unroll
                     for (int j = 0; j < BSPT; ++j) {
                         const u32 sj = c0 + l0 + j;
                         m1[j] = sj < S ? dvals[sj] : 0u;
-                        tops |= (sj < S ? (u32)dtop[sj] : 0u) << (8 * j);
+                        tw |= (sj < S ? (u64)((const uint16_t*)dtop)[sj] : 0ull) << (16 * j);
                     }
-                    if (POS) load_doff<BETA>(doff, c0 + l0, S, pk);
                 }
+#pragma unroll
+                for (int j = 0; j < BSPT; ++j) pk[j] = (u32)(tw >> (16 * j + 8)) & 0xFFu;
 #pragma unroll
                 for (int j = 0; j < BSPT; ++j) {
                     d[j][0] = m1[j];
-                    d[j][BETA - 1] = ((tops >> (8 * j)) & 0xFFu) << 24;
+                    d[j][BETA - 1] = ((u32)(tw >> (16 * j)) & 0xFFu) << 24;
                 }
             } else if (full) {
                 load_deleg<BETA, true>(dvals, c0 + l0, S, d);

@@ -29,8 +29,9 @@ struct SampleArgs {
     u32* ghist;        // fused histogram (fuse == 1)
     int hshift, bins;  // fused histogram: bin = delegate >> hshift, bins bins
     Ctrl* ctrl;
-    // beta 2, staged: when set, dvals[s] holds the first delegate and dtop[s] the top byte of the
-    // second (all a beta-path threshold on the top 8-bit digit needs of it)
+    // beta 2, staged: when set, dvals[s] holds the first delegate and the 16-bit word s of dtop
+    // the top byte of the second (all a beta-path threshold on the top 8-bit digit needs of it)
+    // and, above it, the first's offset in the subrange (0xFF without positions); dpos unused
     u8* dtop = nullptr;
 };
 hipError_t launch_sample(const SampleArgs& a, hipStream_t st);
@@ -107,7 +108,7 @@ struct BetaGatherArgs {
     u32* cnt;
     u32* ghist2;
     int hshift2, bins2;
-    const u8* dtop = nullptr;  // beta 2: delegates as the sampler's dtop form (dvals[s] first, dtop[s] top byte of the second)
+    const u8* dtop = nullptr;  // beta 2: delegates in the sampler's dtop form (dvals[s] first, 16-bit dtop words)
 };
 #ifndef DRTOPK_BG_BSPT
 #define DRTOPK_BG_BSPT 4  // 4 measured faster than 8 (fewer VGPRs, more resident workgroups)

@@ -136,19 +136,20 @@ __device__ __forceinline__ void store_delegates(const LT& t, u64 s, int alpha, u
     if constexpr (std::is_same<LT, Top2P>::value) {
         // both delegates with one 8-byte store (or the first and the second's top byte), the
         // first maximum's offset as one byte
+        const u32 off = t.p1 == 0xFFFFFFFFu ? 0xFFu : (u32)(t.p1 - (u32)(s << alpha)) & 0xFFu;
         if (dtop) {
+            // compact form: the second delegate's top byte and the offset in one 16-bit word
             dvals[s] = t.m1;
-            dtop[s] = (u8)(t.m2 >> 24);
+            ((uint16_t*)dtop)[s] = (uint16_t)((t.m2 >> 24) | (off << 8));
         } else {
             *(uint2*)(dvals + 2 * s) = make_uint2(t.m1, t.m2);
+            ((uint8_t*)dpos)[s] = (uint8_t)off;
         }
-        const u32 off = t.p1 == 0xFFFFFFFFu ? 0xFFu : (u32)(t.p1 - (u32)(s << alpha)) & 0xFFu;
-        ((uint8_t*)dpos)[s] = (uint8_t)off;
         return;
     }
     if (BETA == 2 && POSM == 0 && dtop) {
         dvals[s] = t.val(0);
-        dtop[s] = (u8)(t.val(1) >> 24);
+        ((uint16_t*)dtop)[s] = (uint16_t)((t.val(1) >> 24) | 0xFF00u);
         return;
     }
     u32 packed = 0;
