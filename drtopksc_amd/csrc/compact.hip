@@ -1070,11 +1070,15 @@ __global__ __launch_bounds__(WG) void k_extract_count(Seg g, SelState* sel, u64*
 // Per chunk of TT keys the outputs are placed relative to the chunk's first slot P0 in 32 bits:
 // with lg / lq the keys > K / == K before a key inside the chunk and rem the ties still to take
 // when the chunk starts, a key > K goes to lg + min(lq, rem) and a key == K with lq < rem to lg + lq.
+// off: the segments' exclusive prefixes of packed (gt, eq) counts, or null: then each
+// workgroup sums the counts `cnt` of the segments before its own (few segments: this replaces
+// a one-workgroup scan launch).
 template <class K, bool DIRECT, bool KEYS, bool IDX>
 __global__ __launch_bounds__(WG) void k_extract_write(Seg g, SelState* sel, u64 k, const u64* __restrict__ off,
-                                                      u32* __restrict__ out_keys, i64* __restrict__ out_idx,
-                                                      Ctrl* ctrl) {
+                                                      const u64* __restrict__ cnt, u32* __restrict__ out_keys,
+                                                      i64* __restrict__ out_idx, Ctrl* ctrl) {
     __shared__ u32 scratch[WG / WAVE];
+    __shared__ u64 red[WG / WAVE];
     __shared__ u32 sk[KEYS ? STAGE : 1];
     __shared__ u32 si[IDX ? STAGE : 1];
     const u32 K0 = sel->result;
@@ -1083,7 +1087,20 @@ __global__ __launch_bounds__(WG) void k_extract_write(Seg g, SelState* sel, u64 
     for (u64 s = blockIdx.x; s < g.nseg; s += gridDim.x) {
         const u64 len = seg_len<DIRECT>(g, s);
         const u32* base = g.keys + s * g.segcap;
-        const u64 o = off[s];
+        u64 o;
+        if (off) {
+            o = off[s];
+        } else {
+            u64 part = 0;  // packed fields add without carries: each total is below 2^31
+            for (u64 t = threadIdx.x; t < s; t += WG) part += cnt[t];
+            part = wave_sum_u64(part);
+            if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = part;
+            __syncthreads();
+            o = 0;
+#pragma unroll
+            for (int w = 0; w < WG / WAVE; ++w) o += red[w];
+            __syncthreads();
+        }
         u64 run_g = unpack_hi(o), run_q = unpack_lo(o);
         for (u64 c0 = 0; c0 < len; c0 += TT) {
             const u64 e0 = c0 + (u64)threadIdx.x * TE;
@@ -1142,11 +1159,16 @@ hipError_t launch_extract(const ExtractArgs& a, hipStream_t st) {
 #undef EXC
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = launch_scan(a.cnt, a.seg.nseg, a.off, nullptr, st);
-    if (e != hipSuccess) return e;
+    // up to 2048 segments every workgroup sums the counts before its segment itself
+    const bool fused = a.seg.nseg <= 2048;
+    if (!fused) {
+        e = launch_scan(a.cnt, a.seg.nseg, a.off, nullptr, st);
+        if (e != hipSuccess) return e;
+    }
+    const u64* off = fused ? nullptr : a.off;
 #define EXW(K, D, KK, II)                                                                                        \
-    hipLaunchKernelGGL((k_extract_write<K, D, KK, II>), dim3(grid), dim3(WG), 0, st, a.seg, a.sel, a.k, a.off,   \
-                       (u32*)a.out_keys, a.out_idx, a.ctrl)
+    hipLaunchKernelGGL((k_extract_write<K, D, KK, II>), dim3(grid), dim3(WG), 0, st, a.seg, a.sel, a.k, off,     \
+                       a.cnt, (u32*)a.out_keys, a.out_idx, a.ctrl)
 #define EXW2(K, D)                                                                                                \
     do {                                                                                                          \
         if (a.out_keys && a.out_idx) EXW(K, D, true, true);                                                       \
