@@ -77,6 +77,18 @@ __device__ __forceinline__ void load_te(const u32* p, u64 e0, u64 m, u32* v) {
     }
 }
 
+// Counts of keys > K and == K of thread-owned runs of TE keys; `whole` (uniform) skips the
+// per-key bounds checks of a chunk that lies inside the segment.
+template <int N>
+__device__ __forceinline__ void count_gt_eq(const u32 (&v)[N], u32 K0, bool whole, u32 nvalid, u32& gt, u32& eq) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const bool in = whole || (u32)j < nvalid;
+        gt += (in && v[j] > K0) ? 1u : 0u;
+        eq += (in && v[j] == K0) ? 1u : 0u;
+    }
+}
+
 // ---------------------------------------------------------------- LDS staging
 // Survivors of a 4096-element chunk go to LDS at their rank, then the workgroup copies the
 // chunk's run out with coalesced stores (a thread's own survivors sit at scattered global
@@ -115,31 +127,33 @@ struct RangeRule {
 
 __global__ __launch_bounds__(WG) void k_range_count(const u32* __restrict__ dvals, u64 S, int fullcmp, SelState* sel,
                                                     u64* __restrict__ cnt) {
-    __shared__ u64 scratch[WG / WAVE + 1];
+    __shared__ u32 scratch[WG / WAVE];
     const RangeRule rr(sel, fullcmp);
     const u64 ntiles = range_tiles(S);
     for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const u64 e0 = t * TT + (u64)threadIdx.x * TE;
+        const bool whole = (t + 1) * TT <= S;  // uniform
+        const u32 nvalid = e0 < S ? (u32)std::min<u64>(S - e0, TE) : 0u;
         u32 v[TE];
         load_te<KeyU32, false>(dvals + e0, e0, S, v);
-        u32 gt = 0, eq = 0;
 #pragma unroll
-        for (int j = 0; j < TE; ++j) {
-            const bool in = e0 + j < S;
-            const u32 p = v[j] & rr.mask;
-            gt += in && p > rr.thr;
-            eq += in && p == rr.thr;
-        }
-        u64 tot;
-        block_excl_scan_u64<WG>(pack2(gt, eq), scratch, &tot);
-        if (threadIdx.x == 0) cnt[t] = tot;
+        for (int j = 0; j < TE; ++j) v[j] &= rr.mask;
+        u32 gt = 0, eq = 0;
+        count_gt_eq(v, rr.thr, whole, nvalid, gt, eq);
+        u32 tot;  // packed (gt << 16) | eq: at most TT = 4096 each
+        block_excl_scan_u32<WG>((gt << 16) | eq, scratch, &tot);
+        if (threadIdx.x == 0) cnt[t] = pack2(tot >> 16, tot & 0xFFFFu);
     }
 }
 
+// Positions inside a tile are 32-bit and relative to its first slot, as in k_extract_write.
+// off: the tiles' exclusive prefixes of packed (gt, eq) counts, or null: each workgroup then sums
+// the counts of the tiles before its own.
 __global__ __launch_bounds__(WG) void k_range_write(const u32* __restrict__ dvals, u64 S, u64 k, int fullcmp,
-                                                    SelState* sel, const u64* __restrict__ off, u32* __restrict__ ids,
-                                                    Ctrl* ctrl) {
-    __shared__ u64 scratch[WG / WAVE + 1];
+                                                    SelState* sel, const u64* __restrict__ off,
+                                                    const u64* __restrict__ cnt, u32* __restrict__ ids, Ctrl* ctrl) {
+    __shared__ u32 scratch[WG / WAVE];
+    __shared__ u64 red[WG / WAVE];
     __shared__ u32 sk[STAGE];
     const RangeRule rr(sel, fullcmp);
     const u64 total_gt = sel->m - sel->below - sel->kcount;
@@ -147,42 +161,50 @@ __global__ __launch_bounds__(WG) void k_range_write(const u32* __restrict__ dval
     const u64 ntiles = range_tiles(S);
     for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const u64 e0 = t * TT + (u64)threadIdx.x * TE;
+        const bool whole = (t + 1) * TT <= S;  // uniform
+        const u32 nvalid = e0 < S ? (u32)std::min<u64>(S - e0, TE) : 0u;
         u32 v[TE];
         load_te<KeyU32, false>(dvals + e0, e0, S, v);
-        u32 gt = 0, eq = 0;
+        u64 base;
+        if (off) {
+            base = off[t];
+        } else {
+            u64 part = 0;  // packed fields add without carries: each total is below 2^31
+            for (u64 i = threadIdx.x; i < t; i += WG) part += cnt[i];
+            part = wave_sum_u64(part);
+            if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = part;
+            __syncthreads();
+            base = 0;
 #pragma unroll
-        for (int j = 0; j < TE; ++j) {
-            const bool in = e0 + j < S;
-            v[j] &= rr.mask;
-            gt += in && v[j] > rr.thr;
-            eq += in && v[j] == rr.thr;
+            for (int w = 0; w < WG / WAVE; ++w) base += red[w];
         }
-        u64 tot;
-        const u64 ex = block_excl_scan_u64<WG>(pack2(gt, eq), scratch, &tot);
-        const u64 base = off[t];
-        // the tile's ids occupy [P0, P1): every id before position P0 comes from earlier tiles
-        const u64 P0 = unpack_hi(base) + std::min<u64>(unpack_lo(base), r1);
-        const u64 P1 = unpack_hi(base) + unpack_hi(tot) + std::min<u64>(unpack_lo(base) + unpack_lo(tot), r1);
-        u64 g = unpack_hi(base) + unpack_hi(ex), q = unpack_lo(base) + unpack_lo(ex);
+#pragma unroll
+        for (int j = 0; j < TE; ++j) v[j] &= rr.mask;
+        u32 gt = 0, eq = 0;
+        count_gt_eq(v, rr.thr, whole, nvalid, gt, eq);
+        u32 tot;
+        const u32 ex = block_excl_scan_u32<WG>((gt << 16) | eq, scratch, &tot);
+        const u64 run_q = unpack_lo(base);
+        const u32 rem = (u32)std::min<u64>(r1 - std::min<u64>(run_q, r1), TT);
+        // the tile's ids occupy [P0, P0 + c): every id before position P0 comes from earlier tiles
+        const u64 P0 = unpack_hi(base) + std::min<u64>(run_q, r1);
+        const u32 c = (tot >> 16) + std::min<u32>(tot & 0xFFFFu, rem);
+        u32 lg = ex >> 16, lq = ex & 0xFFFFu;
 #pragma unroll
         for (int j = 0; j < TE; ++j) {
-            if (e0 + j >= S) break;
-            u64 pos = ~0ull;
-            if (v[j] > rr.thr) {
-                pos = g + (q < r1 ? q : r1);
-                ++g;
-            } else if (v[j] == rr.thr) {
-                if (q < r1) pos = g + q;
-                ++q;
-            }
-            if (pos != ~0ull) sk[pos - P0] = (u32)(e0 + j);
+            const bool in = whole || (u32)j < nvalid;
+            const bool isg = in && v[j] > rr.thr, ise = in && v[j] == rr.thr;
+            const u32 sl = isg ? lg + min(lq, rem) : lg + lq;
+            if (isg || (ise && lq < rem)) sk[sl] = (u32)(e0 + j);
+            lg += isg ? 1u : 0u;
+            lq += ise ? 1u : 0u;
         }
         __syncthreads();
-        const u32 c = (u32)(P1 - P0);
-        if (P1 > k) {
+        if (P0 + c > k) {
             if (threadIdx.x == 0) atomicOr(&ctrl->error, ERR_CAPACITY);
         } else {
-            for (u32 i = threadIdx.x; i < c; i += WG) ids[P0 + i] = sk[i];
+            u32* __restrict__ o = ids + P0;
+            for (u32 i = threadIdx.x; i < c; i += WG) o[i] = sk[i];
         }
         __syncthreads();
     }
@@ -195,10 +217,14 @@ hipError_t launch_range(const RangeArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_range_count, dim3(grid), dim3(WG), 0, st, a.dvals, a.S, a.fullcmp, a.sel, a.cnt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = launch_scan(a.cnt, nt, a.off, nullptr, st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_range_write, dim3(grid), dim3(WG), 0, st, a.dvals, a.S, a.k, a.fullcmp, a.sel, a.off,
-                       a.sel_ids, a.ctrl);
+    // up to 2048 tiles every workgroup sums the counts before its tile itself
+    const bool fused = nt <= 2048;
+    if (!fused) {
+        e = launch_scan(a.cnt, nt, a.off, nullptr, st);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_range_write, dim3(grid), dim3(WG), 0, st, a.dvals, a.S, a.k, a.fullcmp, a.sel,
+                       fused ? nullptr : a.off, a.cnt, a.sel_ids, a.ctrl);
     return hipGetLastError();
 }
 
@@ -1032,18 +1058,6 @@ hipError_t launch_seg_total(const u32* cnt, u64 nseg, Ctrl* ctrl, hipStream_t st
 template <bool DIRECT>
 __device__ __forceinline__ u64 seg_len(const Seg& g, u64 s) {
     return DIRECT ? std::min<u64>(g.segcap, g.n - s * g.segcap) : (u64)g.cnt[s];
-}
-
-// Counts of keys > K and == K of thread-owned runs of TE keys; `whole` (uniform) skips the
-// per-key bounds checks of a chunk that lies inside the segment.
-template <int N>
-__device__ __forceinline__ void count_gt_eq(const u32 (&v)[N], u32 K0, bool whole, u32 nvalid, u32& gt, u32& eq) {
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const bool in = whole || (u32)j < nvalid;
-        gt += (in && v[j] > K0) ? 1u : 0u;
-        eq += (in && v[j] == K0) ? 1u : 0u;
-    }
 }
 
 template <class K, bool DIRECT>
