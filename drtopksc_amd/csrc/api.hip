@@ -384,7 +384,10 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
                               dvals, doff, ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins, dtop};
             HIPCHK(launch_beta_gather(ba, st));
         }
-        HIPCHK(launch_seg_total(scnt, pl.nseg, ctrl, st));
+        // with the first digit's histogram fused into the concatenation, the first select also
+        // totals the segments; otherwise a one-workgroup pass does before the first histogram
+        const bool total_in_select = radix && fuse_hist;
+        if (!total_in_select) HIPCHK(launch_seg_total(scnt, pl.nseg, ctrl, st));
         HIPCHK(mark(3));
         // ---- second top-k over the concatenation (radix_select_inplace :1891 / bucket_select :1355)
         const Seg cseg{skeys, sidx, scnt, pl.segcap, pl.nseg, 0};
@@ -392,7 +395,8 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
         if (radix) {
             for (int d = 0; d <= D0; ++d) {
                 if (!(d == 0 && fuse_hist)) HIPCHK(launch_hist_seg(0, cseg, sec, hist, bins, gc, st));
-                HIPCHK(launch_select(0, hist, bins, sec, ctrl, st));
+                if (d == 0 && total_in_select) HIPCHK(launch_select(0, hist, bins, sec, ctrl, st, scnt, pl.nseg));
+                else HIPCHK(launch_select(0, hist, bins, sec, ctrl, st));
             }
         } else {
             HIPCHK(launch_minmax_seg(cseg, sec, gc, st));

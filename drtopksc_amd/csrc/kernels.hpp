@@ -48,7 +48,10 @@ hipError_t launch_hist_raw(int mode, int f32, const u32* keys, SelState* sel, u3
                            unsigned grid, hipStream_t st);
 hipError_t launch_hist_seg(int mode, const Seg& seg, SelState* sel, u32* ghist, int bins, unsigned grid,
                            hipStream_t st);
-hipError_t launch_select(int mode, u32* ghist, int bins, SelState* sel, Ctrl* ctrl, hipStream_t st);
+// segcnt / nseg: first pass over a segmented concatenation; the kernel then also sets its size
+// (concat_size, sel->m and the too-short error) from the segment counts.
+hipError_t launch_select(int mode, u32* ghist, int bins, SelState* sel, Ctrl* ctrl, hipStream_t st,
+                         const u32* segcnt = nullptr, u64 nseg = 0);
 hipError_t launch_bucket_begin(SelState* sel, Ctrl* ctrl, int from_ctrl_minmax, hipStream_t st);
 hipError_t launch_minmax(const u32* arr, SelState* sel, Ctrl* ctrl, unsigned grid, hipStream_t st);
 hipError_t launch_minmax_raw(int f32, const u32* keys, SelState* sel, Ctrl* ctrl, unsigned grid,

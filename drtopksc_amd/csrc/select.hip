@@ -162,9 +162,30 @@ hipError_t launch_hist_seg(int mode, const Seg& seg, SelState* sel, u32* ghist, 
 constexpr int SEL_T = 256;
 
 template <int MODE>
-__global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState* sel, Ctrl* ctrl) {
+__global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState* sel, Ctrl* ctrl,
+                                                 const u32* __restrict__ segcnt, u64 nseg) {
     __shared__ u64 scratch[SEL_T / WAVE + 1];
     __shared__ u64 found[3];  // bin, count before it, count in it
+    u64 seg_m = 0;  // with segcnt: the concatenation's size (every thread holds it)
+    if (segcnt) {
+        // first pass over a segmented concatenation: its size from the segment counts first
+        // (what a separate one-workgroup total would do, see k_seg_total)
+        u64 s = 0;
+        for (u64 i = threadIdx.x; i < nseg; i += SEL_T) s += segcnt[i];
+        u64 tot;
+        block_excl_scan_u64<SEL_T>(s, scratch, &tot);
+        seg_m = tot;
+        if (threadIdx.x == 0) {
+            ctrl->concat_size = tot + ctrl->lonely_dropped;  // the dropped lonely delegates count too
+            sel->m = tot;
+            if (tot < sel->k) {
+                atomicOr(&ctrl->error, ERR_CONCAT_SHORT);
+                sel->done = 1;
+            }
+        }
+        __syncthreads();
+        if (tot < sel->k) return;
+    }
     if (sel->done) return;
     constexpr int MAXPER = MAX_LDS_BINS / SEL_T;  // 16
     const int per = (bins + SEL_T - 1) / SEL_T;
@@ -193,7 +214,8 @@ __global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState
     }
     u64 tot;
     const u64 excl = block_excl_scan_u64<SEL_T>(sum, scratch, &tot);
-    const u64 m = sel->m, k = sel->k, below = sel->below;
+    // sel->m as just written by thread 0 need not be visible yet to the others: use seg_m
+    const u64 m = segcnt ? seg_m : sel->m, k = sel->k, below = sel->below;
     const u64 T = m - k + 1 - below;
     if (threadIdx.x == 0) found[0] = ~0ull;
     __syncthreads();
@@ -269,9 +291,10 @@ __global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState
     }
 }
 
-hipError_t launch_select(int mode, u32* ghist, int bins, SelState* sel, Ctrl* ctrl, hipStream_t st) {
-    if (mode == 0) hipLaunchKernelGGL(k_select<0>, dim3(1), dim3(SEL_T), 0, st, ghist, bins, sel, ctrl);
-    else hipLaunchKernelGGL(k_select<1>, dim3(1), dim3(SEL_T), 0, st, ghist, bins, sel, ctrl);
+hipError_t launch_select(int mode, u32* ghist, int bins, SelState* sel, Ctrl* ctrl, hipStream_t st,
+                         const u32* segcnt, u64 nseg) {
+    if (mode == 0) hipLaunchKernelGGL(k_select<0>, dim3(1), dim3(SEL_T), 0, st, ghist, bins, sel, ctrl, segcnt, nseg);
+    else hipLaunchKernelGGL(k_select<1>, dim3(1), dim3(SEL_T), 0, st, ghist, bins, sel, ctrl, segcnt, nseg);
     return hipGetLastError();
 }
 
