@@ -316,12 +316,13 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
             HIPCHK(launch_find(keys, pl.f32 ? 1 : 0, sec, g, st));
         }
         HIPCHK(mark(3));
-        if (want_out) {
+        if (want_out) {  // the extraction also writes the k-th key and the counts
             ExtractArgs ea{Seg{keys, nullptr, nullptr, DIRECT_TILE, pl.ext_nseg, pl.n}, 1, pl.f32, pl.k, sec, ctrl,
-                           d_topk_keys, d_topk_idx, ecnt, eoff};
+                           d_topk_keys, d_topk_idx, ecnt, eoff, true, d_kth};
             HIPCHK(launch_extract(ea, st));
+        } else {
+            HIPCHK(launch_finalize(sec, ctrl, pl.f32, d_kth, pl.k, st));
         }
-        HIPCHK(launch_finalize(sec, ctrl, pl.f32, d_kth, pl.k, st));
         HIPCHK(mark(4));
     } else {
         SelState* fst = &ctrl->first;
@@ -408,11 +409,12 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
             HIPCHK(launch_find_seg(cseg, sec, gc, st));
         }
         HIPCHK(mark(4));
-        if (want_out) {
-            ExtractArgs ea{cseg, 0, pl.f32, pl.k, sec, ctrl, d_topk_keys, d_topk_idx, ecnt, eoff};
+        if (want_out) {  // the extraction also writes the k-th key and the counts
+            ExtractArgs ea{cseg, 0, pl.f32, pl.k, sec, ctrl, d_topk_keys, d_topk_idx, ecnt, eoff, true, d_kth};
             HIPCHK(launch_extract(ea, st));
+        } else {
+            HIPCHK(launch_finalize(sec, ctrl, pl.f32, d_kth, pl.k, st));
         }
-        HIPCHK(launch_finalize(sec, ctrl, pl.f32, d_kth, pl.k, st));
     }
     HIPCHK(mark(5));
 

@@ -1090,7 +1090,7 @@ __global__ __launch_bounds__(WG) void k_extract_count(Seg g, SelState* sel, u64*
 template <class K, bool DIRECT, bool KEYS, bool IDX>
 __global__ __launch_bounds__(WG) void k_extract_write(Seg g, SelState* sel, u64 k, const u64* __restrict__ off,
                                                       const u64* __restrict__ cnt, u32* __restrict__ out_keys,
-                                                      i64* __restrict__ out_idx, Ctrl* ctrl) {
+                                                      i64* __restrict__ out_idx, Ctrl* ctrl, int fin, u32* d_kth) {
     __shared__ u32 scratch[WG / WAVE];
     __shared__ u64 red[WG / WAVE];
     __shared__ u32 sk[KEYS ? STAGE : 1];
@@ -1098,6 +1098,12 @@ __global__ __launch_bounds__(WG) void k_extract_write(Seg g, SelState* sel, u64 
     const u32 K0 = sel->result;
     const u64 total_gt = sel->m - sel->below - sel->kcount;
     const u64 r = k - total_gt;  // ties taken, lowest index first
+    if (fin && blockIdx.x == 0 && threadIdx.x == 0) {  // k_finalize's work, saving its launch
+        ctrl->kth = K0;
+        ctrl->count_gt = total_gt;
+        ctrl->ties = r;
+        if (d_kth) *d_kth = K::from_ord(K0);
+    }
     for (u64 s = blockIdx.x; s < g.nseg; s += gridDim.x) {
         const u64 len = seg_len<DIRECT>(g, s);
         const u32* base = g.keys + s * g.segcap;
@@ -1182,7 +1188,7 @@ hipError_t launch_extract(const ExtractArgs& a, hipStream_t st) {
     const u64* off = fused ? nullptr : a.off;
 #define EXW(K, D, KK, II)                                                                                        \
     hipLaunchKernelGGL((k_extract_write<K, D, KK, II>), dim3(grid), dim3(WG), 0, st, a.seg, a.sel, a.k, off,     \
-                       a.cnt, (u32*)a.out_keys, a.out_idx, a.ctrl)
+                       a.cnt, (u32*)a.out_keys, a.out_idx, a.ctrl, a.finalize ? 1 : 0, (u32*)a.d_kth)
 #define EXW2(K, D)                                                                                                \
     do {                                                                                                          \
         if (a.out_keys && a.out_idx) EXW(K, D, true, true);                                                       \

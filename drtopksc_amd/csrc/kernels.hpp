@@ -136,6 +136,8 @@ struct ExtractArgs {
     i64* out_idx;
     u64* cnt;          // [nseg] packed (greater, equal)
     u64* off;
+    bool finalize = false;  // also do k_finalize's work (k-th key out, greater / tie counts)
+    void* d_kth = nullptr;
 };
 constexpr u64 DIRECT_TILE = 16384;
 hipError_t launch_extract(const ExtractArgs& a, hipStream_t st);
