@@ -54,14 +54,24 @@ __device__ __forceinline__ u32 bucket_hi(u64 b, u32 lo, u32 hi, u64 B) {
 // ---------------------------------------------------------------- element iteration
 // Calls f(v) for this workgroup's share of a flat array (grid-stride, 16-byte loads) or of
 // a segmented array (one segment per workgroup at a time).
+// Four 16-byte loads per thread are issued before any is used (indices clamped in bounds, the
+// extra ones masked), so a thread keeps four in flight instead of one.
+constexpr int HU = 4;
 template <class F>
 __device__ __forceinline__ void for_flat(const u32* arr, u64 m, F&& f) {
     const u64 nv = m >> 2;
     const u32x4* a4 = (const u32x4*)arr;
-    const u64 stride = (u64)gridDim.x * WG;
-    for (u64 i = (u64)blockIdx.x * WG + threadIdx.x; i < nv; i += stride) {
-        const u32x4 r = a4[i];
-        f(r.x); f(r.y); f(r.z); f(r.w);
+    const u64 stride = (u64)gridDim.x * WG * HU;
+    for (u64 i0 = (u64)blockIdx.x * WG * HU + threadIdx.x; i0 < nv; i0 += stride) {
+        u32x4 r[HU];
+#pragma unroll
+        for (int u = 0; u < HU; ++u) {
+            const u64 i = i0 + (u64)u * WG;
+            r[u] = a4[i < nv ? i : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < HU; ++u)
+            if (i0 + (u64)u * WG < nv) { f(r[u].x); f(r[u].y); f(r[u].z); f(r[u].w); }
     }
     if (blockIdx.x == 0)
         for (u64 i = (nv << 2) + threadIdx.x; i < m; i += WG) f(arr[i]);
@@ -73,9 +83,16 @@ __device__ __forceinline__ void for_seg(const Seg& g, F&& f) {
         const u32* base = g.keys + s * g.segcap;  // segcap is a multiple of 4: 16-byte aligned
         const u64 nv = len >> 2;
         const u32x4* a4 = (const u32x4*)base;
-        for (u64 i = threadIdx.x; i < nv; i += WG) {
-            const u32x4 r = a4[i];
-            f(r.x); f(r.y); f(r.z); f(r.w);
+        for (u64 i0 = threadIdx.x; i0 < nv; i0 += (u64)WG * HU) {
+            u32x4 r[HU];
+#pragma unroll
+            for (int u = 0; u < HU; ++u) {
+                const u64 i = i0 + (u64)u * WG;
+                r[u] = a4[i < nv ? i : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < HU; ++u)
+                if (i0 + (u64)u * WG < nv) { f(r[u].x); f(r[u].y); f(r[u].z); f(r[u].w); }
         }
         for (u64 i = (nv << 2) + threadIdx.x; i < len; i += WG) f(base[i]);
     }
