@@ -970,14 +970,36 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
 // alpha <= 4 the random 16-byte subrange reads are the limit and 5 do better.
 static int beta_waves(int alpha) { return alpha >= 5 ? 8 : 5; }
 
+// Resident workgroups of the beta gather on the current device, from the occupancy of one fixed
+// instantiation (uint32 keys, beta 2, positions, fused histogram) so that planning and launch
+// agree whatever the call's outputs; cached per device.
+template <int W, u32 KB>
+static u64 beta_slots_t() {
+    static int cache[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cache[dev] == 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, reinterpret_cast<const void*>(&k_beta_gather<KeyU32, 2, true, true, W, KB>), WG, 0) != hipSuccess ||
+            nb < 1)
+            nb = W;  // one workgroup per wave slot of a SIMD
+        cache[dev] = nb;
+    }
+    return (u64)cache[dev] * device_cus();
+}
+static u64 beta_slots(int alpha) {
+    return beta_waves(alpha) == 8 ? beta_slots_t<8, 1024>() : beta_slots_t<5, DRTOPK_BG_KBUF>();
+}
+
 // Chunks per beta-gather segment.  A workgroup runs one segment at a time, so equal segments
-// finish in rounds of the resident workgroups (waves per SIMD x CUs); once segments would
-// exceed 16 chunks' worth of rounds, one segment per resident workgroup avoids a partly
-// filled last round (2^30 keys, alpha 4: 4096 segments = 3.2 rounds of 1280).
+// finish in rounds of the resident workgroups; once segments would exceed 16 chunks' worth of
+// rounds, one segment per resident workgroup avoids a partly filled last round (2^30 keys,
+// alpha 4: 4096 segments would be 2.7 rounds of the 1536 resident workgroups).
 static u64 beta_cps(u64 nch, int alpha) {
     const u64 cps = chunks_per_segment(nch);
 #if DRTOPK_BG_ONEROUND
-    const u64 slots = (u64)device_cus() * (u64)beta_waves(alpha);
+    const u64 slots = beta_slots(alpha);
     if (nch > slots * cps) return (nch + slots - 1) / slots;
 #else
     (void)alpha;
