@@ -507,7 +507,9 @@ extern "C" int orc_drtopk(int algo, const u32* keys, u64 n, u64 k, const drtopk_
         // ---- second top-k over the concatenation ----
         int passes = 0;
         const u64 C = ck.size();
-        if (C < k) return 2;  // cannot happen (see the proof in DESIGN.md); reported as an internal error
+        // cannot happen: kth(V) >= kth(delegates) >= flag, and every key >= flag is in the
+        // concatenation (a selected subrange or a lonely delegate); reported as an internal error
+        if (C < k) return 2;
         if (algo == DRTOPK_ALGO_RADIX) K = radix_inplace(ck.data(), C, k, nb, &passes);   // :1891
         else if (algo == DRTOPK_ALGO_BUCKET) K = bucket_kth(ck.data(), C, C - k, B, &passes);  // :1355
         else {
