@@ -210,7 +210,7 @@ def test_full_size_uniform(orc, dt):
     n = 1 << 30
     v = orc.generate(0, 2, n)
     dev = to_dev(v)
-    for k in (1, 1 << 10, 1 << 20, 1 << 24):
+    for k in (1, 1 << 10, 1 << 20, 1 << 22, 1 << 24):
         print(f"[case] full-size uniform k={k}", flush=True)
         kth, tk, ti, st = dt.select(dev, k, stats=True)
         torch.cuda.synchronize()
