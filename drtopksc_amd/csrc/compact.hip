@@ -24,6 +24,7 @@
 //  k_extract_count / k_extract_write  findTopk + FillTopk (MultiGPU/radixselect.cuh:
 //        1355-1383): keys above the k-th largest plus the lowest-index keys equal to it.
 #include <algorithm>
+#include <atomic>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -975,18 +976,18 @@ static int beta_waves(int alpha) { return alpha >= 5 ? 8 : 5; }
 // agree whatever the call's outputs; cached per device.
 template <int W, u32 KB>
 static u64 beta_slots_t() {
-    static int cache[64] = {};
+    static std::atomic<int> cache[64];  // 0 until set; calls from several threads store equal values
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if (cache[dev] == 0) {
-        int nb = 0;
+    int nb = cache[dev].load(std::memory_order_relaxed);
+    if (nb == 0) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &nb, reinterpret_cast<const void*>(&k_beta_gather<KeyU32, 2, true, true, W, KB>), WG, 0) != hipSuccess ||
             nb < 1)
             nb = W;  // one workgroup per wave slot of a SIMD
-        cache[dev] = nb;
+        cache[dev].store(nb, std::memory_order_relaxed);
     }
-    return (u64)cache[dev] * device_cus();
+    return (u64)nb * device_cus();
 }
 static u64 beta_slots(int alpha) {
     return beta_waves(alpha) == 8 ? beta_slots_t<8, 1024>() : beta_slots_t<5, DRTOPK_BG_KBUF>();

@@ -12,6 +12,7 @@
 //  *_seg         the same passes over a segmented concatenation (compact.hip).
 // Range selection, concatenation and top-k extraction live in compact.hip.
 #include <algorithm>
+#include <atomic>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -421,15 +422,17 @@ hipError_t launch_to_user(const u32* ord, u64 m, int f32, void* out, hipStream_t
 }
 
 unsigned device_cus() {
-    static unsigned cache[64] = {0};
+    static std::atomic<unsigned> cache[64];  // 0 until set; concurrent callers store equal values
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (!cache[dev]) {
+    unsigned c = cache[dev].load(std::memory_order_relaxed);
+    if (!c) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
-        cache[dev] = (unsigned)v;
+        c = (unsigned)v;
+        cache[dev].store(c, std::memory_order_relaxed);
     }
-    return cache[dev];
+    return c;
 }
 
 }  // namespace drtopk
