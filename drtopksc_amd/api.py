@@ -106,6 +106,14 @@ def lib():
         L.drtopk_select_batched.restype = ctypes.c_int
         L.drtopk_tuning_default.argtypes = [ctypes.POINTER(Tuning)]
         L.drtopk_tuning_default.restype = None
+        L.drtopk_workspace_check.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_uint32)]
+        L.drtopk_workspace_check.restype = ctypes.c_int
+        L.drtopk_testing_set_segment_cap.argtypes = [u64]
+        L.drtopk_testing_set_segment_cap.restype = ctypes.c_int
+        L.drtopk_file_count.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64)]
+        L.drtopk_file_count.restype = ctypes.c_int
+        L.drtopk_file_read.argtypes = [ctypes.c_char_p, u64, u64, vp, ctypes.c_int, vp]
+        L.drtopk_file_read.restype = ctypes.c_int
         L.drtopk_last_error.argtypes = []
         L.drtopk_last_error.restype = ctypes.c_char_p
         L.drtopk_version.argtypes = []
@@ -191,6 +199,48 @@ def sample_delegates(keys, alpha: int, beta: int = 1, shuffle: bool = False, wan
                                        _stream_ptr(stream))
     _check(rc)
     return vals, pos
+
+
+def workspace_check(workspace, stream=None) -> int:
+    """drtopk_workspace_check: device-side error bits of every call on this workspace since the
+    previous check (cleared by this call); raises DrTopKError (INTERNAL) when any is set."""
+    bits = ctypes.c_uint32()
+    ptr = workspace if isinstance(workspace, int) else workspace.data_ptr()
+    _check(lib().drtopk_workspace_check(ctypes.c_void_p(ptr), _stream_ptr(stream), ctypes.byref(bits)))
+    return int(bits.value)
+
+
+def testing_set_segment_cap(cap: int) -> None:
+    """Testing only: bound every later concatenation segment to `cap` entries (0: full capacity)."""
+    _check(lib().drtopk_testing_set_segment_cap(int(cap)))
+
+
+def file_count(path: str) -> int:
+    """Whole 32-bit values in a raw binary key file."""
+    n = ctypes.c_uint64()
+    _check(lib().drtopk_file_count(os.fsencode(path), ctypes.byref(n)))
+    return int(n.value)
+
+
+def file_read(path: str, first: int = 0, count: int | None = None, out=None, stream=None):
+    """Values [first, first + count) of a raw binary file of 32-bit little-endian integers (the
+    reference's real-dataset format) as keys: into `out` (a numpy uint32 array or a device tensor of
+    4-byte elements), or a new numpy uint32 array when out is None."""
+    import numpy as np
+    if count is None:
+        count = file_count(path) - first
+    if out is None:
+        out = np.empty(count, np.uint32)
+    if isinstance(out, np.ndarray):
+        if out.dtype.itemsize != 4 or out.size < count or not out.flags.c_contiguous:
+            raise DrTopKError(1, "out must be a contiguous array of >= count 4-byte elements")
+        _check(lib().drtopk_file_read(os.fsencode(path), first, count, ctypes.c_void_p(out.ctypes.data), 0, None))
+    else:
+        if out.element_size() != 4 or out.numel() < count or not out.is_cuda or not out.is_contiguous():
+            raise DrTopKError(1, "out must be a contiguous device tensor of >= count 4-byte elements")
+        _check(lib().drtopk_file_read(os.fsencode(path), first, count, ctypes.c_void_p(out.data_ptr()), 1,
+                                      _stream_ptr(stream)))
+    return out
 
 
 def tuning_default() -> Tuning:

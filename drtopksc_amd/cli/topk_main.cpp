@@ -3,9 +3,13 @@
 //
 //   drtopk_topk <log2|V|> <k> <NBitsPerDigit> <beta> [--algo radix|bucket|bitonic] [--alpha A]
 //               [--flags F] [--seed S] [--check] [--buckets B] [--reps R] [--topk] [--copy]
+//               [--file PATH]
 //
 // Keys are uint32 drawn from std::mt19937(seed) (the reference seeds from std::random_device,
-// main.cu:75-76).  alpha defaults to the reference policy (main.cu:83).  --check sorts a host
+// main.cu:75-76), or with --file the first 2^log2|V| values of a raw binary file of 32-bit
+// little-endian integers, the reference's real-dataset format (RealDatasets/BinaryDataset/
+// main.cu:109-115, its fifth argument), read straight to the device with drtopk_file_read.
+// alpha defaults to the reference policy (main.cu:83).  --check sorts a host
 // copy and compares vec1[n-k] with the result, as the reference's Enabletest block does
 // (main.cu:281-296); the exit status is 0 on success, -1 (255) on a failed check or a bad argument.
 // --reps R repeats the timed call R times (phase times of the last one are printed); --topk also
@@ -26,7 +30,7 @@
 int main(int argc, char** argv) {
     if (argc < 5) {
         std::printf("./exe num_element(log2) k NBitsPerDigit beta [--algo radix|bucket|bitonic] [--alpha A] "
-                    "[--flags F] [--seed S] [--check] [--buckets B] [--reps R] [--topk] [--copy]\n");
+                    "[--flags F] [--seed S] [--check] [--buckets B] [--reps R] [--topk] [--copy] [--file PATH]\n");
         return -1;
     }
     const int num_pow = std::atoi(argv[1]);
@@ -38,6 +42,7 @@ int main(int argc, char** argv) {
     unsigned flags = DRTOPK_FLAGS_DEFAULT, seed = 1;
     bool check = false, topk = false, copy = false;
     int reps = 1;
+    std::string file;
     for (int i = 5; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--algo" && i + 1 < argc) algo = argv[++i];
@@ -49,17 +54,27 @@ int main(int argc, char** argv) {
         else if (a == "--topk") topk = true;
         else if (a == "--copy") copy = true;
         else if (a == "--reps" && i + 1 < argc) reps = std::max(1, std::atoi(argv[++i]));
+        else if (a == "--file" && i + 1 < argc) file = argv[++i];
         else { std::printf("wrong input: %s\n", a.c_str()); return -1; }
     }
     if (num_pow < 1 || num_pow > 30 || k < 1 || k > (1ll << num_pow)) { std::printf("wrong input\n"); return -1; }
     const unsigned long long n = 1ull << num_pow;
     std::printf("num_element: %llu\n", n);
     std::vector<unsigned> vec(n);
-    std::mt19937 gen(seed);
-    for (auto& v : vec) v = gen();
     unsigned* vec_d = nullptr;
     if (hipMalloc(&vec_d, n * 4) != hipSuccess) { std::printf("hipMalloc failed\n"); return -1; }
-    if (hipMemcpy(vec_d, vec.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess) { std::printf("H2D failed\n"); return -1; }
+    if (!file.empty()) {
+        if (drtopk_file_read(file.c_str(), 0, n, vec_d, 1, nullptr) != DRTOPK_OK) {
+            std::printf("error: %s\n", drtopk_last_error());
+            return -1;
+        }
+        if (hipMemcpy(vec.data(), vec_d, n * 4, hipMemcpyDeviceToHost) != hipSuccess) { std::printf("D2H failed\n"); return -1; }
+        std::printf("read %llu keys from %s\n", n, file.c_str());
+    } else {
+        std::mt19937 gen(seed);
+        for (auto& v : vec) v = gen();
+        if (hipMemcpy(vec_d, vec.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess) { std::printf("H2D failed\n"); return -1; }
+    }
 
     drtopk_params p;
     drtopk_params_default(&p);

@@ -9,8 +9,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -42,6 +44,9 @@ void clear_error() { g_last_error.clear(); }
 namespace {
 
 constexpr u64 MAX_N = (1ull << 31) - 1;
+
+// drtopk_testing_set_segment_cap: entries a concatenation segment may hold (0: its capacity).
+std::atomic<u64> g_test_segcap{0};
 
 u64 real_delegates(u64 n, int alpha, int beta) {
     u64 L = 1ull << alpha, S = cdiv(n, L), last = n - (S - 1) * L;
@@ -370,19 +375,20 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
         HIPCHK(mark(2));
         // ---- range selection + concatenation (segmented, one segment per tile)
         u32* h2 = fuse_hist ? hist : nullptr;
+        const u64 seg_limit = g_test_segcap.load(std::memory_order_relaxed);
         u32* sidx_or_null = want_out ? sidx : nullptr;
         if (pl.path == 1) {
             RangeArgs ra{dvals, pl.S, pl.k, radix ? 0 : 1, fst, ctrl, sel_ids, rcnt, roff};
             HIPCHK(launch_range(ra, st));
             GatherArgs ga{keys, pl.n, pl.alpha, pl.f32, (pl.flags & DRTOPK_FLAG_FILTER) ? 1 : 0, sel_ids, pl.k,
-                          ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins};
+                          ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins, seg_limit};
             HIPCHK(launch_gather(ga, st));
         } else {
             // filter bit 1: keep keys >= threshold; bit 2 (radix family): lonely delegates below the
             // threshold are counted, not written
             const int bfilter = (pl.flags & DRTOPK_FLAG_FILTER) ? (radix ? 3 : 1) : 0;
             BetaGatherArgs ba{keys, pl.n, pl.alpha, pl.beta, pl.f32, bfilter,
-                              dvals, doff, ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins, dtop};
+                              dvals, doff, ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins, dtop, seg_limit};
             HIPCHK(launch_beta_gather(ba, st));
         }
         // with the first digit's histogram fused into the concatenation, the first select also
@@ -447,9 +453,40 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
             for (int i = 0; i < 6; ++i) hipEventDestroy(ev[i]);
         }
         *h_stats = o;
-        if (hc.error)
-            return fail(DRTOPK_ERR_INTERNAL, "device-side check failed (code 0x%x)", hc.error);
+        if (hc.head.error)
+            return fail(DRTOPK_ERR_INTERNAL, "device-side check failed (code 0x%x)", hc.head.error);
     }
+    return DRTOPK_OK;
+}
+
+drtopk_status drtopk_workspace_check(const void* d_workspace, void* stream, uint32_t* h_error) {
+    clear_error();
+    if (h_error) *h_error = 0;
+    if (!d_workspace || ((uintptr_t)d_workspace & 255) != 0)
+        return fail(DRTOPK_ERR_INVALID_ARG, "d_workspace must be a 256-byte aligned device pointer");
+    hipStream_t st = (hipStream_t)stream;
+    WsHead h;
+    HIPCHK(hipMemcpyAsync(&h, d_workspace, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (h.magic != WS_MAGIC) return fail(DRTOPK_ERR_INVALID_ARG, "the workspace holds no call's result");
+    const u32 bits = h.sticky;  // every failed check also sets the call's own word
+    if (h_error) *h_error = bits;
+    if (h.sticky) {  // read and clear: the next check reports only calls made after this one
+        static const u32 zero = 0;
+        HIPCHK(hipMemcpyAsync((char*)d_workspace + offsetof(WsHead, sticky), &zero, sizeof(zero),
+                              hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    if (bits)
+        return fail(DRTOPK_ERR_INTERNAL, "device-side check failed in a call on this workspace (bits 0x%x:%s%s)",
+                    bits, (bits & ERR_CONCAT_SHORT) ? " concatenation shorter than k" : "",
+                    (bits & ERR_CAPACITY) ? " a buffer bound was exceeded" : "");
+    return DRTOPK_OK;
+}
+
+drtopk_status drtopk_testing_set_segment_cap(uint64_t cap) {
+    clear_error();
+    g_test_segcap.store(cap, std::memory_order_relaxed);
     return DRTOPK_OK;
 }
 

@@ -138,13 +138,23 @@ __device__ __forceinline__ void bsample_tile(const u32* __restrict__ src, u64 le
     }
 }
 
+// Workspace of a batched call: the error words (WsHead, as in every workspace), then the row
+// queue counter of k_brows, then the delegate and selected-id regions (plan offsets).
+struct BatchWs {
+    WsHead head;
+    u32 queue;
+};
+
 __global__ __launch_bounds__(BS_NT) void k_bsample(const unsigned char* __restrict__ plan, const u32* __restrict__ keys,
-                                                   u32* __restrict__ wsq, u32* __restrict__ deleg) {
+                                                   BatchWs* wsh, u32* __restrict__ deleg) {
     const BatchHeader* h = (const BatchHeader*)plan;
     const BatchRow* rows = (const BatchRow*)(plan + h->o_rows);
     const u32* crow = (const u32*)(plan + h->o_chunk_row);
     const u64 nch = h->nchunks;
-    if (blockIdx.x == 0 && threadIdx.x == 0) wsq[0] = 0;  // row queue of k_brows (next launch)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // the call's first kernel resets the head
+        ws_head_begin(&wsh->head);
+        wsh->queue = 0;  // row queue of k_brows (next launch)
+    }
     __shared__ u32 part[1 << (MAX_TILE_LOG2 - 8)];
     for (u64 c = blockIdx.x; c < nch; c += gridDim.x) {
         const BatchRow* rp = rows + crow[c];
@@ -405,7 +415,7 @@ __device__ u64 wg_ordered(const Src& src, u64 E, u32 K, u64 ties, RowLds& L, F e
 }
 
 __global__ __launch_bounds__(BR_NT) void k_brows(const unsigned char* __restrict__ plan, const u32* __restrict__ keys,
-                                                 u32* __restrict__ wsq, const u32* __restrict__ deleg,
+                                                 BatchWs* wsh, const u32* __restrict__ deleg,
                                                  u32* __restrict__ selws, u32* __restrict__ kth_out,
                                                  u32* __restrict__ out_keys, i64* __restrict__ out_idx) {
     const BatchHeader* h = (const BatchHeader*)plan;
@@ -417,7 +427,7 @@ __global__ __launch_bounds__(BR_NT) void k_brows(const unsigned char* __restrict
     const int t = threadIdx.x;
     for (;;) {
         __syncthreads();
-        if (t == 0) L.misc[15] = atomicAdd(wsq, 1u);
+        if (t == 0) L.misc[15] = atomicAdd(&wsh->queue, 1u);
         __syncthreads();
         const u32 q = L.misc[15];
         if (q >= nrows) break;
@@ -452,6 +462,11 @@ __global__ __launch_bounds__(BR_NT) void k_brows(const unsigned char* __restrict
             }
         });
         __syncthreads();
+        // every key of the top-k set is >= T, so the concatenation holds at least k keys
+        if (C < k) {
+            if (t == 0) ws_fail(&wsh->head, ERR_CONCAT_SHORT);
+            continue;
+        }
         // second top-k and extraction
         u64 gt = 0;
         u32 K;
@@ -645,13 +660,13 @@ drtopk_status drtopk_select_batched(const void* h_plan, const void* d_plan, cons
         return fail(DRTOPK_ERR_INVALID_ARG, "misaligned output pointer");
     hipStream_t st = (hipStream_t)stream;
     unsigned char* ws = (unsigned char*)d_workspace;
-    u32* wsq = (u32*)ws;
+    BatchWs* wsh = (BatchWs*)ws;
     u32* deleg = (u32*)(ws + hd.o_ws_deleg);
     u32* selws = (u32*)(ws + hd.o_ws_sel);
     const unsigned char* P = (const unsigned char*)d_plan;
-    hipLaunchKernelGGL(k_bsample, dim3(hd.sample_grid), dim3(BS_NT), 0, st, P, (const u32*)d_keys, wsq, deleg);
+    hipLaunchKernelGGL(k_bsample, dim3(hd.sample_grid), dim3(BS_NT), 0, st, P, (const u32*)d_keys, wsh, deleg);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_brows, dim3(hd.rows_grid), dim3(BR_NT), 0, st, P, (const u32*)d_keys, wsq,
+    hipLaunchKernelGGL(k_brows, dim3(hd.rows_grid), dim3(BR_NT), 0, st, P, (const u32*)d_keys, wsh,
                        (const u32*)deleg, selws, (u32*)d_kth, (u32*)d_topk_keys, (i64*)d_topk_idx);
     HIPCHK(hipGetLastError());
     return DRTOPK_OK;

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/drtopk.h"
+
 typedef uint8_t u8;
 typedef uint32_t u32;
 typedef uint64_t u64;
@@ -63,8 +65,35 @@ struct SelState {
     u32 max_passes;  // bucket: finished after this many levels (result = k-bucket lower bound)
 };
 
+// First 16 bytes of every workspace (single-vector and batched): the device-side error words.
+// `error` holds the bits of the call that last used the workspace; `sticky` accumulates the
+// bits of every call since the last drtopk_workspace_check, which reads and clears it.  The
+// first kernel of a call sets `magic` and clears `sticky` when `magic` does not hold WS_MAGIC
+// yet (a workspace fresh from the allocator).
+constexpr u64 WS_MAGIC = 0x4b504f5452445753ull;  // "SWDRTOPK"
+struct WsHead {
+    u64 magic;
+    u32 error;
+    u32 sticky;
+};
+static_assert(sizeof(WsHead) == 16, "workspace head layout");
+
+// Resets a workspace head at the start of a call (one thread).
+__device__ __forceinline__ void ws_head_begin(WsHead* h) {
+    const u32 sticky = h->magic == WS_MAGIC ? h->sticky : 0u;
+    h->magic = WS_MAGIC;
+    h->error = 0;
+    h->sticky = sticky;
+}
+// Records a failed device-side check in both words.
+__device__ __forceinline__ void ws_fail(WsHead* h, u32 code) {
+    atomicOr(&h->error, code);
+    atomicOr(&h->sticky, code);
+}
+
 // Pipeline control block (one per call, at the start of the workspace).
 struct Ctrl {
+    WsHead head;          // error words (must stay first)
     SelState first;       // first top-k over delegates
     SelState second;      // second top-k over the concatenation (or over V)
     u64 selected;         // selected subranges
@@ -75,14 +104,14 @@ struct Ctrl {
     u64 count_gt;         // keys > k-th largest
     u64 ties;             // keys == k-th largest in the top-k set
     u32 kth;              // ordered bits of the k-th largest
-    u32 error;            // nonzero: a device-side check failed
-    u32 pad[14];
+    u32 pad[15];
 };
 
 enum : u32 {
-    ERR_CONCAT_SHORT = 2u,    // concatenation smaller than k (cannot happen for valid input)
-    ERR_CAPACITY = 4u         // a buffer bound was exceeded
+    ERR_CONCAT_SHORT = DRTOPK_DEVICE_ERR_CONCAT_SHORT,  // concatenation smaller than k (not for valid input)
+    ERR_CAPACITY = DRTOPK_DEVICE_ERR_CAPACITY           // a buffer bound was exceeded
 };
+__device__ __forceinline__ void ctrl_fail(Ctrl* c, u32 code) { ws_fail(&c->head, code); }
 
 // ---------------------------------------------------------------- wave helpers
 __device__ __forceinline__ u32 wave_max_u32(u32 v) {

@@ -202,7 +202,7 @@ __global__ __launch_bounds__(WG) void k_range_write(const u32* __restrict__ dval
         }
         __syncthreads();
         if (P0 + c > k) {
-            if (threadIdx.x == 0) atomicOr(&ctrl->error, ERR_CAPACITY);
+            if (threadIdx.x == 0) ctrl_fail(ctrl, ERR_CAPACITY);
         } else {
             u32* __restrict__ o = ids + P0;
             for (u32 i = threadIdx.x; i < c; i += WG) o[i] = sk[i];
@@ -260,7 +260,7 @@ template <class K, bool HIST2, bool IDX>
 __global__ __launch_bounds__(WG) void k_gather_small(const u32* __restrict__ keys, u64 n, int alpha, int filter, int vec_ok,
                                                const u32* __restrict__ sel_ids, u64 nsel, u64 cps, Ctrl* ctrl,
                                                u32* __restrict__ skeys, u32* __restrict__ sidx, u32* __restrict__ cnt,
-                                               u32* ghist2, int hshift2, int bins2) {
+                                               u32* ghist2, int hshift2, int bins2, u32 cap) {
     __shared__ u64 scratch[WG / WAVE + 1];
     __shared__ u32 lh[HIST2 ? 256 : 1];
     __shared__ u32 sk[STAGE];
@@ -313,6 +313,11 @@ __global__ __launch_bounds__(WG) void k_gather_small(const u32* __restrict__ key
             }
             u64 tot;
             u32 slot = (u32)block_excl_scan_u64<WG>((u64)__popc(take), scratch, &tot);
+            if (run + tot > cap) {  // uniform: the segment would overflow its bound
+                if (threadIdx.x == 0) ctrl_fail(ctrl, ERR_CAPACITY);
+                take = 0;
+                tot = 0;
+            }
 #pragma unroll
             for (int j = 0; j < TE; ++j) {
                 if (take & (1u << j)) {
@@ -394,7 +399,7 @@ template <class K, bool HIST2, bool IDX, bool VEC>
 __global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64 n64, int alpha, int filter, int vec_ok,
                                                const u32* __restrict__ sel_ids, u64 nsel, u64 cps64, Ctrl* ctrl,
                                                u32* __restrict__ skeys, u32* __restrict__ sidx, u32* __restrict__ cnt,
-                                               u32* ghist2, int hshift2, int bins2) {
+                                               u32* ghist2, int hshift2, int bins2, u32 cap) {
     __shared__ u32 scratch[WG / WAVE];
     __shared__ u32 lh[HIST2 ? 256 : 1];
     __shared__ u32 sk[GSTAGE];
@@ -442,6 +447,11 @@ __global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64
             if (ch + 1 < ce) gather_load_keys<VEC>(keys, n, V, alpha, ch + 1, ib, kb);
             u32 tot;
             u32 slot = block_excl_scan_u32<WG>((u32)__popc(take), scratch, &tot);
+            if (run + tot > cap) {  // uniform: the segment would overflow its bound
+                if (threadIdx.x == 0) ctrl_fail(ctrl, ERR_CAPACITY);
+                take = 0;
+                tot = 0;
+            }
             const u64 gpos = (u64)s * segcap + run;
             const bool staged = tot <= GSTAGE;  // uniform; else survivors are stored directly
             u32* __restrict__ gk = skeys + gpos;
@@ -481,8 +491,10 @@ hipError_t launch_gather(const GatherArgs& a, hipStream_t st) {
     const u64 nseg = (nch + cps - 1) / cps;
     const unsigned grid = (unsigned)std::min<u64>(nseg, CGRID);
     const int vec_ok = ((uintptr_t)a.keys & 15) == 0;
+    const u64 segcap = cps * TT;
+    const u32 cap = (u32)std::min<u64>(segcap, a.seg_limit ? a.seg_limit : segcap);
 #define G_ARGS a.keys, a.n, a.alpha, a.filter, vec_ok, a.sel_ids, a.nsel, cps, a.ctrl, a.skeys, a.sidx, a.cnt, \
-               a.ghist2, a.hshift2, a.bins2
+               a.ghist2, a.hshift2, a.bins2, cap
 #define G_LAUNCH(K, H, I)                                                                       \
     do {                                                                                        \
         if (a.alpha >= 2 && vec_ok && a.n >= 4)                                                 \
@@ -606,7 +618,7 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
                                                     const u32* __restrict__ dvals, const void* __restrict__ doff,
                                                     u64 cps64, Ctrl* ctrl, u32* __restrict__ skeys,
                                                     u32* __restrict__ sidx, u32* __restrict__ cnt, u32* ghist2,
-                                                    int hshift2, int bins2, const u8* __restrict__ dtop) {
+                                                    int hshift2, int bins2, const u8* __restrict__ dtop, u32 cap) {
     __shared__ u32 scratch[WG / WAVE];
     __shared__ u32 lh[HIST2 ? 256 : 1];
     __shared__ u64 red[4][WG / WAVE];
@@ -842,11 +854,17 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
             }
             u32 tot;
             const u32 ex = block_excl_scan_u32<WG>(c, scratch, &tot);
+            const bool over = run + tot > cap;  // uniform: the segment would overflow its bound
+            if (over) {
+                if (threadIdx.x == 0) ctrl_fail(ctrl, ERR_CAPACITY);
+                tot = 0;
+            }
             const bool staged = tot <= BSTAGE;  // uniform
             const u64 gbase = (u64)sg * segcap + run;
             u32* __restrict__ gk = skeys + gbase;
             u32* __restrict__ gi = POS ? sidx + gbase : nullptr;
             auto put = [&](u32 slot, u32 o, u32 gidx) {
+                if (over) return;
                 if (staged) {
                     sk[slot] = o;
                     if (POS) si[slot] = gidx;
@@ -1024,8 +1042,10 @@ static hipError_t launch_beta_t(const BetaGatherArgs& a, hipStream_t st) {
     const u64 cps = beta_cps(nch, a.alpha);
     const u64 nseg = (nch + cps - 1) / cps;
     const unsigned grid = (unsigned)std::min<u64>(nseg, CGRID);
+    const u64 segcap = cps * (BETA_TILE_SUBS << a.alpha);
+    const u32 cap = (u32)std::min<u64>(segcap, a.seg_limit ? a.seg_limit : segcap);
 #define B_ARGS a.keys, a.n, a.alpha, a.filter, a.dvals, a.doff, cps, a.ctrl, a.skeys, a.sidx, a.cnt, a.ghist2, \
-               a.hshift2, a.bins2, BETA == 2 ? a.dtop : nullptr
+               a.hshift2, a.bins2, BETA == 2 ? a.dtop : nullptr, cap
     const bool pos = a.sidx != nullptr, h = a.ghist2 != nullptr;
 #define B_LAUNCH(W, KB)                                                                                          \
     do {                                                                                                         \
@@ -1067,7 +1087,7 @@ __global__ __launch_bounds__(SCAN_T) void k_seg_total(const u32* __restrict__ cn
         ctrl->concat_size = tot + ctrl->lonely_dropped;  // the dropped lonely delegates count too
         ctrl->second.m = tot;
         if (tot < ctrl->second.k) {
-            atomicOr(&ctrl->error, ERR_CONCAT_SHORT);
+            ctrl_fail(ctrl, ERR_CONCAT_SHORT);
             ctrl->second.done = 1;  // nothing sensible to select; the call reports the error
         }
     }
@@ -1178,7 +1198,7 @@ __global__ __launch_bounds__(WG) void k_extract_write(Seg g, SelState* sel, u64 
             }
             __syncthreads();
             if (P0 + c > k) {
-                if (threadIdx.x == 0) atomicOr(&ctrl->error, ERR_CAPACITY);
+                if (threadIdx.x == 0) ctrl_fail(ctrl, ERR_CAPACITY);
             } else {
                 u32* __restrict__ ok = KEYS ? out_keys + P0 : nullptr;
                 i64* __restrict__ oi = IDX ? out_idx + P0 : nullptr;

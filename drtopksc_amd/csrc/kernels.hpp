@@ -93,6 +93,7 @@ struct GatherArgs {
     u32* cnt;          // [nseg]
     u32* ghist2;       // fused top-digit histogram of the concatenation, or nullptr
     int hshift2, bins2;
+    u64 seg_limit = 0; // entries a segment may hold (0: its capacity); beyond it the error is raised
 };
 hipError_t launch_gather(const GatherArgs& a, hipStream_t st);
 u64 gather_segcap(u64 V);   // segment capacity / count for V = nsel << alpha virtual keys
@@ -112,6 +113,7 @@ struct BetaGatherArgs {
     u32* ghist2;
     int hshift2, bins2;
     const u8* dtop = nullptr;  // beta 2: delegates in the sampler's dtop form (dvals[s] first, 16-bit dtop words)
+    u64 seg_limit = 0;         // entries a segment may hold (0: its capacity)
 };
 #ifndef DRTOPK_BG_BSPT
 #define DRTOPK_BG_BSPT 4  // 4 measured faster than 8 (fewer VGPRs, more resident workgroups)

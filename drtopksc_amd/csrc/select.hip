@@ -1,7 +1,8 @@
 // select.hip -- the device-resident selection machinery of Dr.Top-k.
 //
-//  k_init        writes the control block and clears histograms and ticket counters
-//                (a kernel, not hipMemsetAsync, so the pipeline can be graph-captured).
+//  k_init        writes the control block (keeping the workspace's sticky error bits) and
+//                clears histograms and counters (a kernel, not hipMemsetAsync, so the pipeline
+//                can be graph-captured).
 //  k_hist        one histogram pass: radix digit of keys matching the current prefix
 //                (CalculateOccurence_inplace[_firsttopk], radixselect.cuh:353-410) or
 //                linear bucket of keys inside the current range (assignBuckets,
@@ -23,7 +24,12 @@ constexpr int MAX_LDS_BINS = 4096;
 
 // ---------------------------------------------------------------- init
 __global__ void k_init(Ctrl* ctrl, Ctrl init, u32* zero, u64 words) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *ctrl = init;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const WsHead h = ctrl->head;  // the sticky error bits survive the call's reset
+        *ctrl = init;
+        ctrl->head = h;
+        ws_head_begin(&ctrl->head);
+    }
     u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) zero[i] = 0;
 }
@@ -197,7 +203,7 @@ __global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState
             ctrl->concat_size = tot + ctrl->lonely_dropped;  // the dropped lonely delegates count too
             sel->m = tot;
             if (tot < sel->k) {
-                atomicOr(&ctrl->error, ERR_CONCAT_SHORT);
+                ctrl_fail(ctrl, ERR_CONCAT_SHORT);
                 sel->done = 1;
             }
         }
@@ -271,7 +277,7 @@ __global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState
         for (int b = b0; b < b1; ++b) ghist[(size_t)r * bins + b] = 0;
     if (threadIdx.x != 0) return;
     if (found[0] == ~0ull) {
-        atomicOr(&ctrl->error, ERR_CONCAT_SHORT);
+        ctrl_fail(ctrl, ERR_CONCAT_SHORT);
         sel->done = 1;
         sel->result = 0;
         return;

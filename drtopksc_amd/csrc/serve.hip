@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstddef>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -112,6 +113,7 @@ struct Slot {
     size_t h_plan_cap = 0;
     u32* h_kth = nullptr;                     // pinned k-th keys, copied back by the launch
     size_t h_kth_cap = 0;
+    u32* h_err = nullptr;                     // pinned device-side error bits of the launch
     drtopk_query solo_q{};                    // the request of a single-vector launch
 };
 
@@ -190,9 +192,12 @@ bool prepare(drtopk_server* s, Slot& sl, Prep& pr) {
     if (pr.solo) {
         pr.take.push_back(head);
     } else {
+        // the small requests ahead of the first large one, in arrival order: a batch never
+        // overtakes a queued large request, so outputs held by a client that waits and releases
+        // in submission order cannot keep that request from ever fitting the arena
         for (u64 t : s->queue) {
-            if (pr.take.size() >= s->cfg.max_batch_rows) break;
-            if (!is_solo(s, s->reqs[t].q)) pr.take.push_back(t);
+            if (pr.take.size() >= s->cfg.max_batch_rows || is_solo(s, s->reqs[t].q)) break;
+            pr.take.push_back(t);
         }
     }
     for (;;) {  // shrink the batch until outputs + scratch fit the arena
@@ -282,6 +287,10 @@ drtopk_status issue(drtopk_server* s, Slot& sl, Prep& pr) {
     }
     if (st == DRTOPK_OK && hipMemcpyAsync(sl.h_kth, A + kth_off, 4 * rows, hipMemcpyDeviceToHost, sl.st) != hipSuccess)
         st = DRTOPK_ERR_HIP;
+    // the launch's device-side error bits: both workspaces start with the same head
+    if (st == DRTOPK_OK && hipMemcpyAsync(sl.h_err, A + pr.scratch_off + offsetof(WsHead, error), sizeof(u32),
+                                          hipMemcpyDeviceToHost, sl.st) != hipSuccess)
+        st = DRTOPK_ERR_HIP;
     if (st == DRTOPK_OK && hipEventRecord(sl.ev, sl.st) != hipSuccess) st = DRTOPK_ERR_HIP;
     return st;
 }
@@ -294,6 +303,7 @@ bool retire(drtopk_server* s, Slot& sl, drtopk_status failed = DRTOPK_OK) {
         const hipError_t q = hipEventQuery(sl.ev);
         if (q == hipErrorNotReady) return false;
         if (q != hipSuccess) st = DRTOPK_ERR_HIP;
+        else if (*sl.h_err) st = DRTOPK_ERR_INTERNAL;  // a device-side check failed
     }
     const u64 t = now_ns();
     for (size_t i = 0; i < sl.tickets.size(); ++i)
@@ -304,6 +314,12 @@ bool retire(drtopk_server* s, Slot& sl, drtopk_status failed = DRTOPK_OK) {
     s->cv_done.notify_all();
     s->cv_work.notify_all();  // arena space and a slot are free again
     return true;
+}
+
+bool any_busy(const drtopk_server* s) {
+    for (const Slot& x : s->slots)
+        if (x.busy) return true;
+    return false;
 }
 
 // One worker per slot: prepares under the lock, issues outside it, so the host cost of
@@ -324,6 +340,15 @@ void worker_main(drtopk_server* s, size_t si) {
         if (s->stop && s->queue.empty()) break;
         Prep pr;
         if (!prepare(s, sl, pr)) {
+            if (s->stop && !s->queue.empty() && !any_busy(s)) {
+                // shutting down, nothing running and the queue head cannot fit: only a client
+                // release could free the arena, and the client is destroying the server
+                const u64 t = now_ns();
+                for (u64 q : s->queue) finish_request(s, q, DRTOPK_ERR_BUDGET, t, 0u);
+                s->queue.clear();
+                s->cv_done.notify_all();
+                continue;
+            }
             s->cv_work.wait_for(lk, std::chrono::milliseconds(1));
             continue;
         }
@@ -411,6 +436,8 @@ drtopk_status drtopk_server_create(const drtopk_server_config* cfg, drtopk_serve
     for (Slot& sl : s->slots) {
         HIPCHK(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+        HIPCHK(hipHostMalloc((void**)&sl.h_err, 256, hipHostMallocDefault));
+        *sl.h_err = 0;
     }
     drtopk_server* raw = s.get();
     for (size_t i = 0; i < raw->slots.size(); ++i) raw->workers.emplace_back(worker_main, raw, i);
@@ -499,6 +526,7 @@ drtopk_status drtopk_server_destroy(drtopk_server* s) {
         if (sl.st) (void)hipStreamDestroy(sl.st);
         if (sl.h_plan) (void)hipHostFree(sl.h_plan);
         if (sl.h_kth) (void)hipHostFree(sl.h_kth);
+        if (sl.h_err) (void)hipHostFree(sl.h_err);
     }
     if (s->d_arena) (void)hipFree(s->d_arena);
     delete s;

@@ -147,6 +147,25 @@ drtopk_status drtopk_sample_delegates(drtopk_dtype dtype, const void* d_keys, ui
                                       int32_t alpha, int32_t beta, uint32_t flags, void* d_vals,
                                       uint32_t* d_pos, void* stream);
 
+/* Device-side checks.  Kernels verify invariants that valid input always satisfies (the
+ * concatenation holds at least k keys; no buffer bound is exceeded).  A failed check sets bits
+ * in the workspace; drtopk_select reports them itself only when h_stats is passed.
+ * drtopk_workspace_check synchronises the stream, reads the bits of every call made on the
+ * workspace (drtopk_select's or drtopk_select_batched's) since the previous check, clears them,
+ * stores them in *h_error (may be NULL) and returns DRTOPK_ERR_INTERNAL when any is set,
+ * DRTOPK_ERR_INVALID_ARG when no call has used the workspace yet.  No reference counterpart:
+ * the reference exits the process on a failed CUDA call (radixselect.cuh:20-28) and checks
+ * nothing on the device. */
+#define DRTOPK_DEVICE_ERR_CONCAT_SHORT 0x2u /* the concatenation held fewer than k keys */
+#define DRTOPK_DEVICE_ERR_CAPACITY     0x4u /* a buffer bound was exceeded */
+drtopk_status drtopk_workspace_check(const void* d_workspace, void* stream, uint32_t* h_error);
+
+/* Testing only: every later drtopk_select call in the process (any thread) lets a segment of its
+ * concatenation hold at most cap entries (0 restores the full capacity).  A segment that would
+ * exceed the bound raises DRTOPK_DEVICE_ERR_CAPACITY instead of being written, which exercises
+ * the device-side checks and their reporting (the result of such a call is not valid). */
+drtopk_status drtopk_testing_set_segment_cap(uint64_t cap);
+
 /* ------------------------------------------------------------------------------------------
  * Batched top-k: many independent rows of uneven length, each with its own k and key type, in
  * two kernel launches.  The reference has no batched entry point; the nearest caller is its
@@ -270,6 +289,21 @@ drtopk_status drtopk_server_replay(drtopk_server* srv, const drtopk_query* q, co
 drtopk_status drtopk_replay_serial(const void* d_store, uint64_t store_len, const drtopk_query* q,
                                    const uint64_t* arrival_ns, uint64_t count, drtopk_result* results,
                                    uint64_t* t0_ns);
+
+/* ------------------------------------------------------------------------------------------
+ * Raw binary key files: the reference's real-dataset format, a bare sequence of 32-bit
+ * little-endian integers with no header, each taken as one key of the same 32 bits
+ * (baseline+filter+beta+shuffle/RealDatasets/BinaryDataset/main.cu:109-115 reads them with
+ * bin_in.read((char*)&value, sizeof(int)) into its unsigned key array).
+ * ------------------------------------------------------------------------------------------ */
+/* Number of whole 32-bit values in the file. */
+drtopk_status drtopk_file_count(const char* path, uint64_t* count);
+/* Values [first, first + count) of the file into dst: host memory, or device memory when
+ * dst_on_device is nonzero (staged through pinned host buffers on `stream`, which is
+ * synchronised before returning).  A file holding fewer values is DRTOPK_ERR_INVALID_ARG (the
+ * reference would keep storing the last value it read). */
+drtopk_status drtopk_file_read(const char* path, uint64_t first, uint64_t count, void* dst, int dst_on_device,
+                               void* stream);
 
 /* Thread-local message for the last error on this thread ("" if none). */
 const char* drtopk_last_error(void);

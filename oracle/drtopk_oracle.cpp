@@ -15,6 +15,8 @@
 //      "ordered" uint32 keys (float32 keys are mapped by sign flip first).
 //   4. The OpenMP CPU baseline timed beside the GPU (nth_element per chunk,
 //      nth_element over the candidates, sort of the top k).
+//   5. An exact O(n) checker of a finished top-k (k-th key, keys, indices) for
+//      full-size inputs and many rows, where nth_element per case is too slow.
 //
 // Deviations from the reference, all forced by exactness or determinism:
 //   - Prefix masks are computed explicitly: a prefix of length 0 is 0.  The
@@ -627,4 +629,90 @@ extern "C" int orc_num_threads(void) {
 #else
     return 1;
 #endif
+}
+
+// ---------------------------------------------------------------------------
+// 5. Exact checker of a finished top-k (tests only).  The k-th largest K is the one key with
+//    #(v > K) < k <= #(v >= K); the top-k set is every index with v > K plus the r = k - #(v > K)
+//    lowest indices with v == K, in ascending index order.  One pass over V counts keys > K and
+//    == K per chunk, a second walks each chunk and compares every taken index with the output at
+//    its position.  Keys, kth and outputs are raw bits of the given type (float32: ordered by the
+//    sign flip).  Returns 0 when (kth, out_keys, out_idx) is exactly that result; otherwise the
+//    first failed condition: 1 bad arguments, 2 K is not the k-th largest, 3 an output index
+//    differs, 4 an output key differs.  out_keys / out_idx may be NULL (not checked).
+// ---------------------------------------------------------------------------
+static int verify_core(const u32* raw, u64 n, u64 k, u32 kth_raw, const u32* out_keys, const i64* out_idx,
+                       int f32, int P) {
+    if (!raw || k == 0 || k > n) return 1;
+    const u32 K = f32 ? f2o(kth_raw) : kth_raw;
+    if (P < 1) P = 1;
+    const u64 chunk = (n + P - 1) / P;
+    std::vector<u64> gtc(P, 0), eqc(P, 0);
+#pragma omp parallel for schedule(static, 1) if (P > 1)
+    for (int c = 0; c < P; ++c) {
+        const u64 b = std::min(n, (u64)c * chunk), e = std::min(n, b + chunk);
+        u64 g = 0, q = 0;
+        for (u64 i = b; i < e; ++i) {
+            const u32 v = f32 ? f2o(raw[i]) : raw[i];
+            g += v > K;
+            q += v == K;
+        }
+        gtc[c] = g;
+        eqc[c] = q;
+    }
+    u64 gt = 0, eq = 0;
+    for (int c = 0; c < P; ++c) { gt += gtc[c]; eq += eqc[c]; }
+    if (!(gt < k && k <= gt + eq)) return 2;
+    if (!out_keys && !out_idx) return 0;
+    const u64 r = k - gt;
+    std::vector<u64> wofs(P + 1, 0), tofs(P + 1, 0);
+    for (int c = 0; c < P; ++c) {
+        tofs[c + 1] = tofs[c] + eqc[c];
+        const u64 tk = tofs[c] >= r ? 0 : std::min(eqc[c], r - tofs[c]);
+        wofs[c + 1] = wofs[c] + gtc[c] + tk;
+    }
+    std::vector<int> bad(P, 0);
+#pragma omp parallel for schedule(static, 1) if (P > 1)
+    for (int c = 0; c < P; ++c) {
+        const u64 b = std::min(n, (u64)c * chunk), e = std::min(n, b + chunk);
+        u64 w = wofs[c], t = tofs[c];
+        for (u64 i = b; i < e && !bad[c]; ++i) {
+            const u32 v = f32 ? f2o(raw[i]) : raw[i];
+            const bool take = v > K || (v == K && t++ < r);
+            if (!take) continue;
+            if (out_idx && out_idx[w] != (i64)i) bad[c] = 3;
+            else if (out_keys && out_keys[w] != raw[i]) bad[c] = 4;
+            ++w;
+        }
+    }
+    for (int c = 0; c < P; ++c)
+        if (bad[c]) return bad[c];
+    return 0;
+}
+
+extern "C" int orc_verify_topk(const u32* keys, u64 n, u64 k, u32 kth_raw, const u32* out_keys, const i64* out_idx,
+                               int f32, int threads) {
+#ifdef _OPENMP
+    const int P = threads > 0 ? threads : omp_get_max_threads();
+#else
+    const int P = 1;
+#endif
+    return verify_core(keys, n, k, kth_raw, out_keys, out_idx, f32, std::max(1, std::min<int>(P, (int)std::min<u64>(n, 4096))));
+}
+
+// The checker over many rows of one key buffer, rows in parallel: row r holds n[r] keys at
+// offset off[r], its k-th key is kth[r] and its k outputs sit at out_off[r].  codes[r] gets the
+// row's result (0 = exact).  Returns the number of rows that failed.
+extern "C" u64 orc_verify_rows(const u32* base, u64 nrows, const u64* off, const u64* n, const u64* k,
+                               const u64* out_off, const int* f32, const u32* kth, const u32* out_keys,
+                               const i64* out_idx, int* codes) {
+    u64 nbad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : nbad)
+    for (i64 r = 0; r < (i64)nrows; ++r) {
+        const int c = verify_core(base + off[r], n[r], k[r], kth[r], out_keys ? out_keys + out_off[r] : nullptr,
+                                  out_idx ? out_idx + out_off[r] : nullptr, f32[r], 1);
+        codes[r] = c;
+        nbad += c != 0;
+    }
+    return nbad;
 }

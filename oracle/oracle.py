@@ -92,6 +92,12 @@ def lib():
         L.orc_cpu_baseline.argtypes = [u32p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, u32p, u32p, i64p]
         L.orc_cpu_baseline.restype = ctypes.c_double
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_verify_topk.argtypes = [u32p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, u32p, i64p,
+                                      ctypes.c_int, ctypes.c_int]
+        L.orc_verify_topk.restype = ctypes.c_int
+        L.orc_verify_rows.argtypes = [u32p, ctypes.c_uint64, u64p, u64p, u64p, u64p, ctypes.POINTER(ctypes.c_int),
+                                      u32p, u32p, i64p, ctypes.POINTER(ctypes.c_int)]
+        L.orc_verify_rows.restype = ctypes.c_uint64
         _lib = L
     return _lib
 
@@ -216,3 +222,39 @@ def cpu_baseline(okeys: np.ndarray, k: int, threads: int = 0):
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+def verify_topk(raw: np.ndarray, k: int, kth_raw: int, out_keys=None, out_idx=None, f32: bool = False,
+                threads: int = 0) -> int:
+    """Exact O(n) check of a finished top-k on raw key bits (float32 bits when f32): 0 when the
+    k-th key, the top-k keys and the indices (either may be None) are exactly the defined result;
+    else 2 (wrong k-th key), 3 (an index differs) or 4 (a key differs)."""
+    raw = np.ascontiguousarray(raw, dtype=np.uint32)
+    ok = np.ascontiguousarray(out_keys, dtype=np.uint32) if out_keys is not None else None
+    oi = np.ascontiguousarray(out_idx, dtype=np.int64) if out_idx is not None else None
+    if (ok is not None and ok.size < k) or (oi is not None and oi.size < k):
+        return 1
+    return int(lib().orc_verify_topk(_p(raw), raw.size, k, int(kth_raw) & 0xFFFFFFFF,
+                                     _p(ok) if ok is not None else None,
+                                     _p(oi, ctypes.c_int64) if oi is not None else None, int(f32), threads))
+
+
+def verify_rows(base: np.ndarray, rows, kth_raw: np.ndarray, out_keys=None, out_idx=None) -> np.ndarray:
+    """verify_topk for every row (offset, n, k, out_offset, dtype) of one buffer, rows in parallel;
+    returns the per-row codes (0 = exact)."""
+    base = np.ascontiguousarray(base, dtype=np.uint32)
+    nr = len(rows)
+    off = np.array([r[0] for r in rows], np.uint64)
+    n = np.array([r[1] for r in rows], np.uint64)
+    k = np.array([r[2] for r in rows], np.uint64)
+    oo = np.array([r[3] for r in rows], np.uint64)
+    f = np.array([r[4] for r in rows], np.int32)
+    kth = np.ascontiguousarray(kth_raw, dtype=np.uint32)
+    ok = np.ascontiguousarray(out_keys, dtype=np.uint32) if out_keys is not None else None
+    oi = np.ascontiguousarray(out_idx, dtype=np.int64) if out_idx is not None else None
+    codes = np.zeros(nr, np.int32)
+    lib().orc_verify_rows(_p(base), nr, _p(off, ctypes.c_uint64), _p(n, ctypes.c_uint64), _p(k, ctypes.c_uint64),
+                          _p(oo, ctypes.c_uint64), _p(f, ctypes.c_int), _p(kth),
+                          _p(ok) if ok is not None else None, _p(oi, ctypes.c_int64) if oi is not None else None,
+                          _p(codes, ctypes.c_int))
+    return codes

@@ -185,3 +185,38 @@ def test_server_validation(L):
     c.d_store = 256
     c.reserved[0] = 1
     assert L.drtopk_server_create(ctypes.byref(c), ctypes.byref(h)) == 1
+
+
+def test_workspace_check_validation(L):
+    """drtopk_workspace_check validates its pointer before any HIP call."""
+    bits = ctypes.c_uint32(7)
+    assert L.drtopk_workspace_check(None, None, ctypes.byref(bits)) == 1
+    assert bits.value == 0
+    assert L.drtopk_workspace_check(ctypes.c_void_p(0x1008), None, None) == 1  # not 256-byte aligned
+    assert "aligned" in L.drtopk_last_error().decode()
+
+
+def test_file_loader_host(L, tmp_path):
+    """Raw 32-bit little-endian key files (the reference's real-dataset format): count, whole and
+    partial reads into host memory, the same bits as written (negative int32 values become keys
+    >= 2^31, as the reference's int -> unsigned store does), and short files rejected."""
+    import numpy as np
+
+    from drtopksc_amd import api
+    rng = np.random.default_rng(5)
+    vals = rng.integers(-2 ** 31, 2 ** 31, 100003, dtype=np.int64).astype("<i4")
+    vals[:4] = [-1, 0, 2 ** 31 - 1, -2 ** 31]
+    f = tmp_path / "scores.bin"
+    f.write_bytes(vals.tobytes() + b"\x01\x02")  # a trailing partial value is not counted
+    assert api.file_count(str(f)) == vals.size
+    got = api.file_read(str(f))
+    assert np.array_equal(got, vals.view(np.uint32))
+    assert got[0] == 0xFFFFFFFF and got[3] == 0x80000000
+    part = api.file_read(str(f), first=17, count=1000)
+    assert np.array_equal(part, vals.view(np.uint32)[17:1017])
+    with pytest.raises(api.DrTopKError) as e:
+        api.file_read(str(f), first=vals.size - 5, count=6)
+    assert e.value.status == 1 and "holds" in str(e.value)
+    with pytest.raises(api.DrTopKError):
+        api.file_count(str(tmp_path / "missing.bin"))
+    assert L.drtopk_file_read(None, 0, 1, None, 0, None) == 1

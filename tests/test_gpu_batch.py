@@ -152,3 +152,44 @@ def test_batch_graph_replay(orc, dt):
         g.replay()
         torch.cuda.synchronize()
         assert torch.equal(out[0], kth) and torch.equal(out[1], tk) and torch.equal(out[2], ti), rep
+
+
+@pytest.mark.timeout(900)
+def test_batch_full_workload(orc, dt):
+    """The whole batched workload: 4096 rows, lengths log-uniform 2^12..2^22 (2.47 G keys), coin-flip
+    uint32 uniform / float32 N(0,1), k log-uniform in [1, min(n/8, 4096)] (the bench's rows and keys).
+    Every row is checked exactly (k-th key, keys, indices) by the O(n) checker; 256 rows, the 16
+    longest among them, also against nth_element."""
+    from drtopksc_amd.workloads import batch_mix, pack
+    mix = batch_mix(seed=7, nrows=4096)
+    rows, keys_len, out_len = pack(mix)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    buf = torch.randint(-2 ** 31, 2 ** 31 - 1, (keys_len,), dtype=torch.int32, device="cuda", generator=g)
+    for off, n, k, oo, d in rows:
+        if d == 1:
+            buf[off:off + n] = torch.randn(n, device="cuda", generator=g).view(torch.int32)
+    print(f"[case] batch full workload rows={len(rows)} keys={keys_len}", flush=True)
+    b = dt.Batch(rows, keys_len, out_len)
+    kth, tk, ti = b.run(buf)
+    torch.cuda.synchronize()
+    host = buf.cpu().numpy().view(np.uint32)
+    kth_h = kth.cpu().numpy().view(np.uint32)
+    tk_h = tk.cpu().numpy().view(np.uint32)
+    ti_h = ti.cpu().numpy()
+    codes = orc.verify_rows(host, rows, kth_h, tk_h, ti_h)
+    bad = np.nonzero(codes)[0]
+    assert bad.size == 0, [(int(r), rows[r], int(codes[r])) for r in bad[:10]]
+    rng = np.random.default_rng(77)
+    longest = np.argsort([-r[1] for r in rows], kind="stable")[:16]
+    sample = np.unique(np.concatenate([longest, rng.choice(len(rows), 240, replace=False)]))
+    for r in sample:
+        off, n, k, oo, d = rows[r]
+        raw = host[off:off + n]
+        o = orc.f32_to_ordered(raw) if d == 1 else raw
+        K, keys, idx = orc.topk_reference(o, k)
+        gk = orc.f32_to_ordered(kth_h[r:r + 1]) if d == 1 else kth_h[r:r + 1]
+        gkeys = orc.f32_to_ordered(tk_h[oo:oo + k]) if d == 1 else tk_h[oo:oo + k]
+        assert int(gk[0]) == K and np.array_equal(ti_h[oo:oo + k], idx) and np.array_equal(gkeys, keys), (r, n, k, d)
+    del buf
+    torch.cuda.empty_cache()

@@ -46,8 +46,8 @@ def compare(orc, dt, name, okeys, dev, k, algo, flags, alpha=-1, beta=2, nb=8, B
     print(f"[case] {name} algo={algo} k={k} alpha={alpha} beta={beta} nb={nb} B={B} flags={flags} "
           f"n={okeys.size} f32={f32}", flush=True)
     po = orc.params(alpha=alpha, beta=beta, digit_bits=nb, num_buckets=B, flags=flags)
-    if orc.resolve(algo, okeys.size, k, po) is None:
-        return False
+    # every listed case must be one the reference's parameter rules accept
+    assert orc.resolve(algo, okeys.size, k, po) is not None, (name, "parameters rejected", k, alpha, beta)
     K, keys, idx, ost = orc.drtopk(algo, okeys, k, po)
     pg = dt.params(alpha=alpha, beta=beta, digit_bits=nb, num_buckets=B, flags=flags)
     gk, gkeys, gidx, gst = run_gpu(dt, dev, k, algo, pg)
@@ -64,7 +64,6 @@ def compare(orc, dt, name, okeys, dev, k, algo, flags, alpha=-1, beta=2, nb=8, B
     if algo == 0 or ost["path"] != 0:
         assert gst["second_passes"] == ost["second_passes"] or (ost["path"] == 0 and okeys.size > (1 << 22)), \
             (name, "second_passes", gst["second_passes"], ost["second_passes"])
-    return True
 
 
 # ------------------------------------------------------------------ delegate pass
@@ -232,6 +231,10 @@ def test_pipeline_bitonic(orc, dt, name):
     dev = to_dev_f32(v) if f32 else to_dev(v)
     for k, alpha in [(1, -1), (2, 10), (16, -1), (128, 12), (1024, 8), (4096, 5), (8192, 6), (1 << 14, 5),
                      (64, 0), (8192, 0)]:
+        # the bitonic family needs more subranges than k (sampleBitonic.cu:217-221): ties8 holds
+        # 300001 keys, too few for 2^alpha-key subranges at (128, 12), (8192, 6) and (16384, 5)
+        if alpha > 0 and -(-v.size >> alpha) <= k:
+            continue
         compare(orc, dt, name, ok, dev, k, 2, orc.FLAGS_DEFAULT, alpha=alpha, f32=f32)
 
 

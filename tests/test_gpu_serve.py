@@ -99,3 +99,58 @@ def test_replay_matches_serial(orc, dt):
             a = orc.f32_to_ordered(a)
         assert res[i].status == 0 and ser[i].status == 0
         assert int(a[0]) == K and int(a[1]) == K, (i, q)
+
+
+@pytest.mark.timeout(900)
+def test_serve_full_trace(orc, dt):
+    """The whole served workload: 2000 requests, n log-uniform 2^10..2^26 (10.7 G keys in one device
+    store), coin-flip uint32 uniform / float32 N(0,1), k log-uniform in [1, min(n/4, 2^16)], default
+    server (64 GiB arena).  All requests are queued at once, then waited for in order; every result
+    (k-th key, keys, indices) is checked exactly by the O(n) checker, 256 of them (the 16 longest
+    among them) also against nth_element.  The same trace replayed with Poisson arrivals at 20 K/s
+    returns the same k-th keys."""
+    from drtopksc_amd.workloads import pack, serve_mix
+    mix = serve_mix(seed=8, nreq=2000, rate_per_s=20000.0)
+    rows, keys_len, _ = pack([(n, k, d) for _, n, k, d in mix])
+    g = torch.Generator(device="cuda")
+    g.manual_seed(8)
+    store = torch.randint(-2 ** 31, 2 ** 31 - 1, (keys_len,), dtype=torch.int32, device="cuda", generator=g)
+    for off, n, k, oo, d in rows:
+        if d == 1:
+            store[off:off + n] = torch.randn(n, device="cuda", generator=g).view(torch.int32)
+    torch.cuda.synchronize()
+    qs = [(off, n, k, d) for off, n, k, oo, d in rows]
+    out_len = sum(q[2] for q in qs)
+    kth = np.zeros(len(qs), np.uint32)
+    okeys = np.zeros(out_len, np.uint32)
+    oidx = np.zeros(out_len, np.int64)
+    print(f"[case] serve full trace requests={len(qs)} keys={keys_len}", flush=True)
+    with dt.Server(store) as srv:
+        tickets = [srv.submit(*q) for q in qs]
+        for i, (t, (off, n, k, oo, d)) in enumerate(zip(tickets, rows)):
+            r = srv.wait(t)
+            kb, keys, idx = srv.fetch(r, k)
+            kth[i] = kb
+            okeys[oo:oo + k] = keys
+            oidx[oo:oo + k] = idx
+            srv.release(t)
+        arr = [int(t * 1e9) for t, _, _, _ in mix]
+        res, _ = srv.replay(qs, arr)
+        assert all(res[i].status == 0 and res[i].kth_bits == kth[i] for i in range(len(qs)))
+    host = store.cpu().numpy().view(np.uint32)
+    codes = orc.verify_rows(host, rows, kth, okeys, oidx)
+    bad = np.nonzero(codes)[0]
+    assert bad.size == 0, [(int(r), rows[r], int(codes[r])) for r in bad[:10]]
+    rng = np.random.default_rng(88)
+    longest = np.argsort([-r[1] for r in rows], kind="stable")[:16]
+    sample = np.unique(np.concatenate([longest, rng.choice(len(rows), 240, replace=False)]))
+    for r in sample:
+        off, n, k, oo, d = rows[r]
+        raw = host[off:off + n]
+        o = orc.f32_to_ordered(raw) if d == 1 else raw
+        K, keys, idx = orc.topk_reference(o, k)
+        gk = orc.f32_to_ordered(kth[r:r + 1]) if d == 1 else kth[r:r + 1]
+        gkeys = orc.f32_to_ordered(okeys[oo:oo + k]) if d == 1 else okeys[oo:oo + k]
+        assert int(gk[0]) == K and np.array_equal(oidx[oo:oo + k], idx) and np.array_equal(gkeys, keys), (r, n, k, d)
+    del store, host
+    torch.cuda.empty_cache()

@@ -335,3 +335,88 @@ def test_cpu_baseline_matches(orc):
         t, K, keys, idx = orc.cpu_baseline(v, k, threads=4)
         K2, keys2, idx2 = np_topk(v, k)
         assert t >= 0 and K == K2 and np.array_equal(idx, idx2) and np.array_equal(keys, keys2)
+
+
+# ---------------------------------------------------------------- exact checker
+@pytest.mark.parametrize("n,k", [(1, 1), (10, 3), (1000, 1), (1000, 999), (4099, 64), (65537, 4096)])
+@pytest.mark.parametrize("kind", ["u32", "ties", "f32"])
+def test_verify_topk_accepts_numpy_result(orc, n, k, kind):
+    """The checker accepts the numpy brute-force top-k (on every thread count) and rejects each
+    planted mistake: a k-th key one off, two indices swapped, a later tie instead of the first
+    untaken one, a changed key."""
+    rng = np.random.default_rng(n + k)
+    if kind == "u32":
+        raw = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+        ordk = raw
+    elif kind == "ties":
+        raw = rng.integers(0, 5, n, dtype=np.uint64).astype(np.uint32)
+        ordk = raw
+    else:
+        raw = np.array([-2.0, -0.0, 0.0, 1.5, -1e-20, 3.0, 1.5, np.inf], np.float32)[
+            rng.integers(0, 8, n)].view(np.uint32)
+        ordk = orc.f32_to_ordered(raw)
+    K, keys_o, idx = np_topk(ordk, k)
+    kth_raw = int(raw[idx[np.nonzero(keys_o == K)[0][0]]]) if np.any(keys_o == K) else None
+    keys_raw = raw[idx]
+    f32 = kind == "f32"
+    for threads in (1, 3, 8):
+        assert orc.verify_topk(raw, k, kth_raw, keys_raw, idx, f32=f32, threads=threads) == 0
+    assert orc.verify_topk(raw, k, kth_raw, None, None, f32=f32) == 0
+    # planted mistakes (in the checker's input, on the CPU)
+    # K + 1 (ordered) has #(v >= K + 1) = #(v > K) < k keys at or above it: never the k-th largest
+    if K < 0xFFFFFFFF:
+        bad_k = orc.ordered_to_f32(np.array([K + 1], np.uint32))[0] if f32 else K + 1
+        assert orc.verify_topk(raw, k, int(bad_k), None, None, f32=f32) == 2
+    # K - 1 has #(v > K - 1) = #(v >= K) >= k keys above it: never the k-th largest either
+    if K > 0:
+        bad_k = orc.ordered_to_f32(np.array([K - 1], np.uint32))[0] if f32 else K - 1
+        assert orc.verify_topk(raw, k, int(bad_k), None, None, f32=f32) == 2
+    if k >= 2:
+        j = idx.copy()
+        j[0], j[1] = j[1], j[0]
+        assert orc.verify_topk(raw, k, kth_raw, keys_raw, j, f32=f32) == 3
+        kk = keys_raw.copy()
+        kk[-1] ^= 1
+        assert orc.verify_topk(raw, k, kth_raw, kk, idx, f32=f32) == 4
+    ties_all = np.nonzero(ordk == K)[0]
+    taken = np.intersect1d(ties_all, idx)
+    if taken.size and taken.size < ties_all.size:
+        j = idx.copy()
+        pos = np.nonzero(j == taken[-1])[0][0]
+        later = ties_all[ties_all > taken[-1]][0]
+        j[pos] = later
+        j.sort()
+        assert orc.verify_topk(raw, k, kth_raw, None, j, f32=f32) == 3
+
+
+def test_verify_rows(orc):
+    """The row-parallel checker: every row of a packed buffer exact; one wrong row is named."""
+    rng = np.random.default_rng(3)
+    rows, parts, off, oo = [], [], 0, 0
+    kth, ok, oi = [], [], []
+    for r in range(40):
+        n = int(rng.integers(1, 5000))
+        k = int(rng.integers(1, n + 1))
+        d = r % 2
+        raw = (rng.standard_normal(n).astype(np.float32).view(np.uint32) if d else
+               rng.integers(0, 50, n, dtype=np.uint64).astype(np.uint32))
+        o = orc.f32_to_ordered(raw) if d else raw
+        K, _, idx = np_topk(o, k)
+        kth.append(int(raw[idx[np.nonzero(o[idx] == K)[0][0]]]))
+        ok.append(raw[idx])
+        oi.append(idx)
+        pad = (-n) % 4
+        parts.append(np.concatenate([raw, np.zeros(pad, np.uint32)]))
+        rows.append((off, n, k, oo, d))
+        off += n + pad
+        oo += k
+    base = np.concatenate(parts)
+    okeys, oidx = np.concatenate(ok), np.concatenate(oi)
+    kth = np.array(kth, np.uint32)
+    assert not np.any(orc.verify_rows(base, rows, kth, okeys, oidx))
+    bad = oidx.copy()
+    r7 = rows[7]
+    if r7[2] >= 2:
+        bad[r7[3]], bad[r7[3] + 1] = bad[r7[3] + 1], bad[r7[3]]
+        codes = orc.verify_rows(base, rows, kth, okeys, bad)
+        assert list(np.nonzero(codes)[0]) == [7]
