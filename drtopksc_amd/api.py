@@ -97,6 +97,9 @@ def lib():
         L.drtopk_select.argtypes = [ctypes.c_int, ctypes.c_int, vp, u64, u64, ctypes.POINTER(Params), vp,
                                     ctypes.c_size_t, vp, vp, vp, ctypes.POINTER(Stats), vp]
         L.drtopk_select.restype = ctypes.c_int
+        L.drtopk_select_static.argtypes = [ctypes.c_int, ctypes.c_int, vp, u64, u64, ctypes.POINTER(Params), vp,
+                                           vp, vp, ctypes.POINTER(Stats), vp]
+        L.drtopk_select_static.restype = ctypes.c_int
         L.drtopk_sample_delegates.argtypes = [ctypes.c_int, vp, u64, i32, i32, u32, vp, vp, vp]
         L.drtopk_sample_delegates.restype = ctypes.c_int
         L.drtopk_batch_plan.argtypes = [ctypes.POINTER(Row), u64, u64, u64, ctypes.POINTER(Tuning), vp,
@@ -182,6 +185,30 @@ def select(keys, k: int, algo: int = ALGO_RADIX, p: Params | None = None, want_k
                              ctypes.byref(st) if stats else None, _stream_ptr(stream))
     _check(rc)
     return kth, tk, ti, (st.as_dict() if stats else None)
+
+
+STATIC_SCRATCH_BYTES = 64 << 20  # DRTOPK_STATIC_SCRATCH_BYTES
+
+
+def select_static(keys, k: int, algo: int = ALGO_RADIX, p: Params | None = None, stats: bool = False, out=None,
+                  stream=None):
+    """select() through drtopk_select_static: no workspace, the library's per-GPU static scratch
+    (problems up to STATIC_SCRATCH_BYTES of workspace).  `out` = (kth, topk_keys or None,
+    topk_idx or None), preallocated by the caller; returns the stats dict or None."""
+    if not keys.is_cuda:
+        raise DrTopKError(1, "keys must be a device tensor")
+    if not keys.is_contiguous():
+        raise DrTopKError(1, "keys must be contiguous")
+    p = p or params()
+    kth, tk, ti = out
+    st = Stats()
+    rc = lib().drtopk_select_static(algo, _dtype_code(keys), ctypes.c_void_p(keys.data_ptr()), keys.numel(), k,
+                                    ctypes.byref(p), ctypes.c_void_p(kth.data_ptr()),
+                                    ctypes.c_void_p(tk.data_ptr()) if tk is not None else None,
+                                    ctypes.c_void_p(ti.data_ptr()) if ti is not None else None,
+                                    ctypes.byref(st) if stats else None, _stream_ptr(stream))
+    _check(rc)
+    return st.as_dict() if stats else None
 
 
 def sample_delegates(keys, alpha: int, beta: int = 1, shuffle: bool = False, want_pos: bool = True, stream=None):

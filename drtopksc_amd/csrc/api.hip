@@ -15,6 +15,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/drtopk.h"
@@ -221,20 +222,14 @@ drtopk_status drtopk_workspace_size(drtopk_algo algo, drtopk_dtype dtype, uint64
     return DRTOPK_OK;
 }
 
-drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_keys, uint64_t n, uint64_t k,
-                            const drtopk_params* p, void* d_workspace, size_t workspace_bytes, void* d_kth,
-                            void* d_topk_keys, int64_t* d_topk_idx, drtopk_stats* h_stats, void* stream) {
-    clear_error();
-    Plan pl;
-    drtopk_status s = resolve(algo, dtype, n, k, p, &pl);
-    if (s != DRTOPK_OK) return s;
-    if (!d_keys) return fail(DRTOPK_ERR_INVALID_ARG, "d_keys is NULL");
-    if (((uintptr_t)d_keys & 15) != 0) return fail(DRTOPK_ERR_INVALID_ARG, "d_keys must be 16-byte aligned");
-    if (!d_workspace) return fail(DRTOPK_ERR_INVALID_ARG, "d_workspace is NULL");
-    if (((uintptr_t)d_workspace & 255) != 0) return fail(DRTOPK_ERR_INVALID_ARG, "d_workspace must be 256-byte aligned");
-    if (workspace_bytes < pl.total)
-        return fail(DRTOPK_ERR_WORKSPACE_TOO_SMALL, "workspace %zu bytes < %llu needed", workspace_bytes,
-                    (unsigned long long)pl.total);
+}  // extern "C"
+
+namespace {
+
+// The whole pipeline of one top-k on a resolved plan, into a workspace of at least pl.total
+// bytes (checked by the caller).
+drtopk_status select_impl(const Plan& pl, const void* d_keys, void* d_workspace, void* d_kth, void* d_topk_keys,
+                          int64_t* d_topk_idx, drtopk_stats* h_stats, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     char* ws = (char*)d_workspace;
     Ctrl* ctrl = (Ctrl*)(ws + pl.o_ctrl);
@@ -457,6 +452,85 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
             return fail(DRTOPK_ERR_INTERNAL, "device-side check failed (code 0x%x)", hc.head.error);
     }
     return DRTOPK_OK;
+}
+
+drtopk_status check_keys(const void* d_keys) {
+    if (!d_keys) return fail(DRTOPK_ERR_INVALID_ARG, "d_keys is NULL");
+    if (((uintptr_t)d_keys & 15) != 0) return fail(DRTOPK_ERR_INVALID_ARG, "d_keys must be 16-byte aligned");
+    return DRTOPK_OK;
+}
+
+// ---- scratch of drtopk_select_static: one file-scope device array per GPU.  In host code the
+// name `scratch` is only a stand-in for the symbol; its device address, different on every
+// GPU, comes from hipGetSymbolAddress and is what every kernel argument uses.
+constexpr size_t STATIC_SCRATCH_BYTES = (size_t)DRTOPK_STATIC_SCRATCH_BYTES;
+__device__ __align__(256) u8 scratch[STATIC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::atomic<void*> g_scratch_ptr[MAX_DEVICES];
+std::mutex g_scratch_mu;
+
+// Device address of the current device's scratch, looked up once per device.
+drtopk_status static_scratch(void** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(DRTOPK_ERR_UNSUPPORTED, "device ordinal %d >= %d", dev, MAX_DEVICES);
+    void* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!p) {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        p = g_scratch_ptr[dev].load(std::memory_order_relaxed);
+        if (!p) {
+            void* q = nullptr;
+            HIPCHK(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+            if (!q || ((uintptr_t)q & 255) != 0)
+                return fail(DRTOPK_ERR_HIP, "static scratch address %p is not 256-byte aligned", q);
+            g_scratch_ptr[dev].store(q, std::memory_order_release);
+            p = q;
+        }
+    }
+    *out = p;
+    return DRTOPK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_keys, uint64_t n, uint64_t k,
+                            const drtopk_params* p, void* d_workspace, size_t workspace_bytes, void* d_kth,
+                            void* d_topk_keys, int64_t* d_topk_idx, drtopk_stats* h_stats, void* stream) {
+    clear_error();
+    Plan pl;
+    drtopk_status s = resolve(algo, dtype, n, k, p, &pl);
+    if (s != DRTOPK_OK) return s;
+    if ((s = check_keys(d_keys)) != DRTOPK_OK) return s;
+    if (!d_workspace) return fail(DRTOPK_ERR_INVALID_ARG, "d_workspace is NULL");
+    if (((uintptr_t)d_workspace & 255) != 0) return fail(DRTOPK_ERR_INVALID_ARG, "d_workspace must be 256-byte aligned");
+    if (workspace_bytes < pl.total)
+        return fail(DRTOPK_ERR_WORKSPACE_TOO_SMALL, "workspace %zu bytes < %llu needed", workspace_bytes,
+                    (unsigned long long)pl.total);
+    return select_impl(pl, d_keys, d_workspace, d_kth, d_topk_keys, d_topk_idx, h_stats, stream);
+}
+
+drtopk_status drtopk_select_static(drtopk_algo algo, drtopk_dtype dtype, const void* d_keys, uint64_t n,
+                                   uint64_t k, const drtopk_params* p, void* d_kth, void* d_topk_keys,
+                                   int64_t* d_topk_idx, drtopk_stats* h_stats, void* stream) {
+    clear_error();
+    Plan pl;
+    drtopk_status s = resolve(algo, dtype, n, k, p, &pl);
+    if (s != DRTOPK_OK) return s;
+    if ((s = check_keys(d_keys)) != DRTOPK_OK) return s;
+    if (pl.total > STATIC_SCRATCH_BYTES)
+        return fail(DRTOPK_ERR_WORKSPACE_TOO_SMALL,
+                    "n = %llu, k = %llu needs %llu bytes of scratch; the static scratch holds %llu "
+                    "(use drtopk_select with a workspace)",
+                    (unsigned long long)n, (unsigned long long)k, (unsigned long long)pl.total,
+                    (unsigned long long)STATIC_SCRATCH_BYTES);
+    if (pl.flags & DRTOPK_FLAG_TIMING)
+        return fail(DRTOPK_ERR_INVALID_ARG, "DRTOPK_FLAG_TIMING creates HIP events; not available here");
+    void* ws = nullptr;
+    if ((s = static_scratch(&ws)) != DRTOPK_OK) return s;
+    return select_impl(pl, d_keys, ws, d_kth, d_topk_keys, d_topk_idx, h_stats, stream);
 }
 
 drtopk_status drtopk_workspace_check(const void* d_workspace, void* stream, uint32_t* h_error) {

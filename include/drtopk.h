@@ -135,6 +135,25 @@ drtopk_status drtopk_select(drtopk_algo algo, drtopk_dtype dtype, const void* d_
                             size_t workspace_bytes, void* d_kth, void* d_topk_keys,
                             int64_t* d_topk_idx, drtopk_stats* h_stats, void* stream);
 
+/* drtopk_select without a workspace argument, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Scratch is a file-scope device array of
+ * DRTOPK_STATIC_SCRATCH_BYTES held by every GPU the library runs on until the process ends; its
+ * address is looked up once per device on the first call there.  The call allocates nothing
+ * (no hipMalloc / hipFree / event creation); the caller still passes keys and outputs.
+ *   - A problem whose drtopk_workspace_size exceeds DRTOPK_STATIC_SCRATCH_BYTES is refused with
+ *     DRTOPK_ERR_WORKSPACE_TOO_SMALL (at 64 MiB: every k for n <= 2^22, and k <= 4096 up to
+ *     n = 2^31-1 on the radix and bucket families).
+ *   - DRTOPK_FLAG_TIMING is refused (DRTOPK_ERR_INVALID_ARG).
+ *   - Every call on one GPU shares that GPU's scratch, which keeps the last call's contents:
+ *     calls that overlap on one GPU (other streams, other host threads) race and must be
+ *     serialised by the caller.  Device-side checks are reported through h_stats only.
+ * Other arguments and results are exactly drtopk_select's. */
+#define DRTOPK_STATIC_SCRATCH_BYTES (64ull << 20)
+drtopk_status drtopk_select_static(drtopk_algo algo, drtopk_dtype dtype, const void* d_keys, uint64_t n,
+                                   uint64_t k, const drtopk_params* p, void* d_kth, void* d_topk_keys,
+                                   int64_t* d_topk_idx, drtopk_stats* h_stats, void* stream);
+
 /* Delegate pass alone.  Replaces sampleMax (radixselect.cuh:883-918; bitonic/sampleBitonic.cu:150-185)
  * for beta == 1 and sampleMax_NoReduction_Beta2/3/4 (radixselect.cuh:967-1202) for beta 2..4.
  * Subrange s = keys[s*2^alpha, min((s+1)*2^alpha, n)); S = ceil(n / 2^alpha).

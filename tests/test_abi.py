@@ -89,6 +89,27 @@ def test_select_rejects_bad_pointers(L):
     assert rc == 2  # workspace too small
 
 
+def test_select_static_validation(L):
+    """drtopk_select_static refuses, before any device work, a problem whose workspace exceeds the
+    static scratch and the timing flag (it would create events)."""
+    from drtopksc_amd import api
+    m = re.search(r"#define DRTOPK_STATIC_SCRATCH_BYTES \((\d+)ull << (\d+)\)", open(HEADER).read())
+    assert m and int(m.group(1)) << int(m.group(2)) == api.STATIC_SCRATCH_BYTES
+    p = api.params()
+    need = ws(L, 0, 1 << 26, 1 << 20)[1]
+    assert need > api.STATIC_SCRATCH_BYTES
+    rc = L.drtopk_select_static(0, 0, ctypes.c_void_p(256), 1 << 26, 1 << 20, ctypes.byref(p), None, None, None,
+                                None, None)
+    assert rc == 2 and "static scratch" in L.drtopk_last_error().decode()
+    assert ws(L, 0, 1 << 22, 1 << 22)[1] <= api.STATIC_SCRATCH_BYTES  # every k fits at n = 2^22
+    pt = api.params(flags=api.FLAGS_DEFAULT | api.FLAG_TIMING)
+    rc = L.drtopk_select_static(0, 0, ctypes.c_void_p(256), 1 << 20, 10, ctypes.byref(pt), None, None, None,
+                                None, None)
+    assert rc == 1 and "TIMING" in L.drtopk_last_error().decode()
+    rc = L.drtopk_select_static(0, 0, ctypes.c_void_p(4), 1 << 20, 10, ctypes.byref(p), None, None, None, None, None)
+    assert rc == 1 and "aligned" in L.drtopk_last_error().decode()
+
+
 def test_version(L):
     assert b"gfx950" in L.drtopk_version()
 
