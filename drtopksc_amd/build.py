@@ -19,6 +19,9 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libdrtopk.so")
 CLI_SRC = os.path.join(HERE, "cli", "topk_main.cpp")
 CLI = os.path.join(LIB_DIR, "drtopk_topk")
+# GPU check of the C++ mirror (include/drtopk.hpp), run by tests/test_gpu_hpp.py
+HPP_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "hpp_mirror.cpp")
+HPP_TEST = os.path.join(os.path.dirname(HERE), "tests", "cpp", "hpp_mirror")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 ARCH = os.environ.get("DRTOPK_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -63,6 +66,13 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"cli build failed:\n{r.stderr}")
     os.replace(CLI + ".tmp", CLI)
+    if os.path.exists(HPP_TEST_SRC):
+        cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-Wno-unused-result", f"-I{INCLUDE}", HPP_TEST_SRC, f"-L{LIB_DIR}", "-ldrtopk",
+               "-Wl,-rpath,$ORIGIN/../../drtopksc_amd/lib", "-o", HPP_TEST + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hpp mirror test build failed:\n{r.stderr}")
+        os.replace(HPP_TEST + ".tmp", HPP_TEST)
     if verbose:
         print(f"built {LIB} and {CLI}")
     return LIB

@@ -7,14 +7,16 @@
 // so every comparison is strict: the top-k is by key, lowest index first among equal keys,
 // and the result does not depend on how the network is scheduled.
 //
-// Scheduling on gfx950:
-//  - k <= BT/2: one workgroup reduces an 8192-composite LDS tile (64 KB) to its own top-k
-//    (sort into k-runs, then log2(T/k) merge+rebuild rounds entirely in LDS); tiles are
-//    reduced level by level until one tile remains.  The reference instead halves once per
-//    launch (bitonic_com.cuh:648-657).
-//  - k > BT/2 (the reference's largerK global-scratch variant): the network runs over a
-//    global buffer; strides >= BT are global passes, all smaller strides of a stage are
-//    fused into one LDS pass per tile.
+// Scheduling on gfx950 (160 KB of LDS per CU):
+//  - k <= T/2: one workgroup reduces an LDS tile of T composites to its own top-k (sort into
+//    k-runs, then log2(T/k) merge+rebuild rounds entirely in LDS); tiles are reduced level by
+//    level until one tile remains.  T = 8192 (64 KB, two workgroups per CU) for k <= 2048 and
+//    T = 16384 (128 KB, one 1024-thread workgroup per CU) above, so k up to 8192 stays in LDS
+//    and each level shrinks the data by T/k >= 2.  The reference instead halves once per launch
+//    (bitonic_com.cuh:648-657).
+//  - k > 8192 (the reference's largerK global-scratch variant): the network runs over a global
+//    buffer; strides >= 16384 are global passes, all smaller strides of a stage are fused into
+//    one LDS pass per 16384-composite tile.
 #include <algorithm>
 
 #include "common.hpp"
@@ -22,8 +24,10 @@
 
 namespace drtopk {
 
-constexpr u32 BT = 8192;      // LDS tile of composites
-constexpr int BTT = 512;      // threads per tile workgroup
+constexpr u32 BT_SMALL = 8192;   // LDS tile of composites for k <= 2048 (64 KB)
+constexpr int BTT_SMALL = 512;
+constexpr u32 BT = 16384;        // LDS tile for larger k and the global network (128 KB)
+constexpr int BTT = 1024;
 
 __device__ __forceinline__ void cmpx(u64* A, u32 i, u32 inc, bool up) {
     const u64 a = A[i], b = A[i + inc];
@@ -35,9 +39,10 @@ __device__ __forceinline__ void cmpx(u64* A, u32 i, u32 inc, bool up) {
 
 // Steps inc = inc0 .. 1 of stage `len` on a tile of T elements whose first global index is
 // `base`; the direction of a pair comes from bit 2*len of its global index.
+template <int NT>
 __device__ __forceinline__ void lds_steps(u64* A, u32 T, u64 base, u64 len, u64 inc0) {
     for (u64 inc = inc0; inc > 0; inc >>= 1) {
-        for (u32 t = threadIdx.x; t < T / 2; t += BTT) {
+        for (u32 t = threadIdx.x; t < T / 2; t += NT) {
             const u32 low = t & (u32)(inc - 1), i = 2 * t - low;
             cmpx(A, i, (u32)inc, ((base + i) & (2 * len)) == 0);
         }
@@ -45,23 +50,24 @@ __device__ __forceinline__ void lds_steps(u64* A, u32 T, u64 base, u64 len, u64 
     }
 }
 
-// ---------------------------------------------------------------- tile reduction (k <= BT/2)
-// Reduces tile b of A (T elements) to its top-k, sorted ascending, at out[b*k, b*k + k).
-__global__ __launch_bounds__(BTT) void k_bt_tile_reduce(const u64* __restrict__ A, u32 T, u32 k,
-                                                        u64* __restrict__ out) {
-    __shared__ u64 S[BT];
+// ---------------------------------------------------------------- tile reduction (k <= TILE/2)
+// Reduces tile b of A (T <= TILE elements) to its top-k, sorted ascending, at out[b*k, b*k + k).
+template <u32 TILE, int NT>
+__global__ __launch_bounds__(NT) void k_bt_tile_reduce(const u64* __restrict__ A, u32 T, u32 k,
+                                                       u64* __restrict__ out) {
+    __shared__ u64 S[TILE];
     const u64 base = (u64)blockIdx.x * T;
-    for (u32 i = threadIdx.x; i < T; i += BTT) S[i] = A[base + i];
+    for (u32 i = threadIdx.x; i < T; i += NT) S[i] = A[base + i];
     __syncthreads();
     // sort into alternating k-runs (tile-local directions)
-    for (u64 len = 1; len < k; len <<= 1) lds_steps(S, T, 0, len, len);
+    for (u64 len = 1; len < k; len <<= 1) lds_steps<NT>(S, T, 0, len, len);
     u32 cur = T;
     while (cur > k) {
         // merge: element t of the halved tile = max of the pair (i, i+k) of runs 2r, 2r+1
-        u64 m[BT / 2 / BTT];
+        u64 m[TILE / 2 / NT];
 #pragma unroll
-        for (u32 j = 0; j < BT / 2 / BTT; ++j) {
-            const u32 t = threadIdx.x + j * BTT;
+        for (u32 j = 0; j < TILE / 2 / NT; ++j) {
+            const u32 t = threadIdx.x + j * NT;
             if (t < cur / 2) {
                 const u32 i = 2 * t - (t & (k - 1));
                 m[j] = S[i] > S[i + k] ? S[i] : S[i + k];
@@ -69,16 +75,16 @@ __global__ __launch_bounds__(BTT) void k_bt_tile_reduce(const u64* __restrict__ 
         }
         __syncthreads();
 #pragma unroll
-        for (u32 j = 0; j < BT / 2 / BTT; ++j) {
-            const u32 t = threadIdx.x + j * BTT;
+        for (u32 j = 0; j < TILE / 2 / NT; ++j) {
+            const u32 t = threadIdx.x + j * NT;
             if (t < cur / 2) S[t] = m[j];
         }
         __syncthreads();
         cur >>= 1;
         // rebuild every run of k into sorted order (alternating)
-        lds_steps(S, cur, 0, k >> 1, k >> 1);
+        lds_steps<NT>(S, cur, 0, k >> 1, k >> 1);
     }
-    for (u32 i = threadIdx.x; i < k; i += BTT) out[(u64)blockIdx.x * k + i] = S[i];
+    for (u32 i = threadIdx.x; i < k; i += NT) out[(u64)blockIdx.x * k + i] = S[i];
 }
 
 // ---------------------------------------------------------------- global network (k > BT/2)
@@ -88,7 +94,7 @@ __global__ __launch_bounds__(BTT) void k_bt_sort_lds(u64* __restrict__ A, u64 le
     const u64 base = (u64)blockIdx.x * BT;
     for (u32 i = threadIdx.x; i < BT; i += BTT) S[i] = A[base + i];
     __syncthreads();
-    for (u64 len = 1; len <= lenmax; len <<= 1) lds_steps(S, BT, base, len, len);
+    for (u64 len = 1; len <= lenmax; len <<= 1) lds_steps<BTT>(S, BT, base, len, len);
     for (u32 i = threadIdx.x; i < BT; i += BTT) A[base + i] = S[i];
 }
 
@@ -98,7 +104,7 @@ __global__ __launch_bounds__(BTT) void k_bt_stage_lds(u64* __restrict__ A, u64 l
     const u64 base = (u64)blockIdx.x * BT;
     for (u32 i = threadIdx.x; i < BT; i += BTT) S[i] = A[base + i];
     __syncthreads();
-    lds_steps(S, BT, base, len, BT / 2);
+    lds_steps<BTT>(S, BT, base, len, BT / 2);
     for (u32 i = threadIdx.x; i < BT; i += BTT) A[base + i] = S[i];
 }
 
@@ -128,25 +134,26 @@ __global__ __launch_bounds__(WG) void k_bt_merge(const u64* __restrict__ A, u64 
 
 static unsigned grid_for(u64 items) { return (unsigned)std::max<u64>(1, std::min<u64>((items + WG - 1) / WG, 8192)); }
 
+// Tile reduction levels with tiles of TILE composites: n -> k.
+template <u32 TILE, int NT>
+u64* tile_levels(u64* A, u64* B, u64 n, u64 k, hipStream_t st) {
+    while (n > TILE) {
+        hipLaunchKernelGGL((k_bt_tile_reduce<TILE, NT>), dim3((unsigned)(n / TILE)), dim3(NT), 0, st, A, TILE,
+                           (u32)k, B);
+        n = (n / TILE) * k;
+        std::swap(A, B);
+    }
+    // last tile: n <= TILE elements -> k (also sorts when n == k)
+    const u32 T = (u32)std::max<u64>(n, 1);
+    hipLaunchKernelGGL((k_bt_tile_reduce<TILE, NT>), dim3(1), dim3(NT), 0, st, A, T, (u32)k, B);
+    return B;
+}
+
 // Top-k (ascending, at A[0..k)) of the n composites in A (n a power of two, n >= k, k a power
 // of two); B is scratch of n elements.  Returns the buffer holding the result.
 u64* bitonic_topk(u64* A, u64* B, u64 n, u64 k, hipStream_t st) {
-    if (k <= BT / 2) {
-        while (n > BT) {
-            hipLaunchKernelGGL(k_bt_tile_reduce, dim3((unsigned)(n / BT)), dim3(BTT), 0, st, A, BT, (u32)k, B);
-            n = (n / BT) * k;
-            std::swap(A, B);
-        }
-        // last tile: n <= BT elements -> k (also sorts when n == k)
-        const u32 T = (u32)std::max<u64>(n, 2 * k) <= BT ? (u32)std::max<u64>(n, 1) : BT;
-        if (n == k) {
-            // already k survivors; sort them ascending with the run sort of one run of k
-            hipLaunchKernelGGL(k_bt_tile_reduce, dim3(1), dim3(BTT), 0, st, A, (u32)k, (u32)k, B);
-        } else {
-            hipLaunchKernelGGL(k_bt_tile_reduce, dim3(1), dim3(BTT), 0, st, A, T, (u32)k, B);
-        }
-        return B;
-    }
+    if (k <= BT_SMALL / 4) return tile_levels<BT_SMALL, BTT_SMALL>(A, B, n, k, st);
+    if (k <= BT / 2) return tile_levels<BT, BTT>(A, B, n, k, st);
     // global network
     hipLaunchKernelGGL(k_bt_sort_lds, dim3((unsigned)(n / BT)), dim3(BTT), 0, st, A, (u64)(BT / 2));
     for (u64 len = BT; len < k; len <<= 1) {

@@ -13,7 +13,9 @@
 //
 // Differences, all deliberate:
 //  - The reference's caller-allocated buffers (Max_d, SubrangeId_d, SelectedSubrangeId_d,
-//    ConcatenatedRange_d, counters; main.cu:205-250) become one drtopk::Workspace.
+//    ConcatenatedRange_d, counters; main.cu:205-250) become one drtopk::Workspace, sized by the
+//    caller before the calls (Workspace::reserve_for); the calls themselves never allocate and
+//    return DRTOPK_ERR_WORKSPACE_TOO_SMALL when it is too small.
 //  - Errors are returned as drtopk_status (the reference prints and exit()s, radixselect.cuh:20-28).
 //  - The reference reports the k-th element only; the optional TopKElements_d / TopKIndices_d
 //    receive the top-k keys and indices (index order, lowest index wins ties).
@@ -31,14 +33,8 @@
 
 namespace drtopk {
 
-template <typename T>
-struct dtype_of;
-template <>
-struct dtype_of<unsigned int> { static constexpr drtopk_dtype value = DRTOPK_U32; };
-template <>
-struct dtype_of<float> { static constexpr drtopk_dtype value = DRTOPK_F32; };
-
-// Device scratch owned by the caller; grows on demand outside the select call.
+// Device scratch owned by the caller.  reserve / reserve_for allocate (hipMalloc); the select
+// calls below only use what has been reserved.
 class Workspace {
    public:
     Workspace() = default;
@@ -56,6 +52,11 @@ class Workspace {
         size_ = bytes;
         return DRTOPK_OK;
     }
+    // Reserve enough for any call of the family `algo` on n keys of type data_t with this k:
+    // every alpha (auto, 0 .. log2 n), beta (1 .. 4), digit width (1 .. 16) and bucket count up
+    // to 4096, plus the slot the k-th key is returned through.
+    template <typename data_t>
+    drtopk_status reserve_for(drtopk_algo algo, uint64_t n, uint64_t k);
     void* data() const { return ptr_; }
     size_t size() const { return size_; }
 
@@ -63,6 +64,36 @@ class Workspace {
     void* ptr_ = nullptr;
     size_t size_ = 0;
 };
+
+template <typename data_t>
+struct dtype_of;
+template <>
+struct dtype_of<unsigned int> { static constexpr drtopk_dtype value = DRTOPK_U32; };
+template <>
+struct dtype_of<float> { static constexpr drtopk_dtype value = DRTOPK_F32; };
+
+template <typename data_t>
+drtopk_status Workspace::reserve_for(drtopk_algo algo, uint64_t n, uint64_t k) {
+    size_t need = 0;
+    bool any = false;
+    for (int nb = 1; nb <= 16; nb *= 2)
+        for (int beta = 1; beta <= 4; ++beta)
+            for (int alpha = -1; alpha <= 31; ++alpha) {
+                drtopk_params p;
+                drtopk_params_default(&p);
+                p.alpha = alpha;
+                p.beta = beta;
+                p.digit_bits = nb;
+                p.num_buckets = 4096;
+                size_t b = 0;
+                if (drtopk_workspace_size(algo, dtype_of<data_t>::value, n, k, &p, &b) == DRTOPK_OK) {
+                    need = b > need ? b : need;
+                    any = true;
+                }
+            }
+    if (!any) return DRTOPK_ERR_INVALID_ARG;
+    return reserve((need + 255) / 256 * 256 + 256);
+}
 
 namespace detail {
 template <typename data_t>
@@ -73,7 +104,7 @@ inline drtopk_status run(drtopk_algo algo, const data_t* vec_d, uint64_t n, uint
     drtopk_status s = drtopk_workspace_size(algo, dtype_of<data_t>::value, n, k, &p, &bytes);
     if (s != DRTOPK_OK) return s;
     const size_t kth_off = (bytes + 255) / 256 * 256;  // the k-th key lives after the workspace
-    if ((s = ws.reserve(kth_off + 256)) != DRTOPK_OK) return s;
+    if (!ws.data() || ws.size() < kth_off + 256) return DRTOPK_ERR_WORKSPACE_TOO_SMALL;
     data_t* kth_d = (data_t*)((char*)ws.data() + kth_off);
     drtopk_stats local;
     s = drtopk_select(algo, dtype_of<data_t>::value, vec_d, n, k, &p, ws.data(), kth_off, kth_d, keys_d, idx_d,

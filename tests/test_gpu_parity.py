@@ -224,15 +224,17 @@ def test_full_size_uniform(orc, dt):
 @pytest.mark.parametrize("name", ["uniform", "ties8", "lognormal", "f32normal", "ragged"])
 def test_pipeline_bitonic(orc, dt, name):
     """Bitonic family (bitonic_firstTopk + bitonic, bitonic_com.cuh:610-728): k a power of two;
-    the LDS tile network (k <= 4096) and the global network (k > 4096)."""
+    the 64 KB LDS tile network (k <= 2048), the 128 KB one (k = 4096, 8192) and the global network
+    (k > 8192)."""
     data = {n: (v, f) for n, v, f in datasets(orc)}
     v, f32 = data[name]
     ok = orc.f32_to_ordered(v) if f32 else v
     dev = to_dev_f32(v) if f32 else to_dev(v)
-    for k, alpha in [(1, -1), (2, 10), (16, -1), (128, 12), (1024, 8), (4096, 5), (8192, 6), (1 << 14, 5),
-                     (64, 0), (8192, 0)]:
+    for k, alpha in [(1, -1), (2, 10), (16, -1), (128, 12), (1024, 8), (2048, 7), (4096, 5), (8192, 6),
+                     (1 << 14, 5), (1 << 15, 4), (64, 0), (8192, 0)]:
         # the bitonic family needs more subranges than k (sampleBitonic.cu:217-221): ties8 holds
-        # 300001 keys, too few for 2^alpha-key subranges at (128, 12), (8192, 6) and (16384, 5)
+        # 300001 keys, too few for 2^alpha-key subranges at (128, 12), (8192, 6), (16384, 5) and
+        # (32768, 4)
         if alpha > 0 and -(-v.size >> alpha) <= k:
             continue
         compare(orc, dt, name, ok, dev, k, 2, orc.FLAGS_DEFAULT, alpha=alpha, f32=f32)
@@ -259,3 +261,24 @@ def test_f32_2p28_bitonic_and_radix(orc, dt, dist, seed):
                 assert 5 <= st["alpha"] <= 16
     del dev
     torch.cuda.empty_cache()
+
+
+def test_customized_vector(orc, dt):
+    """The reference's adversarial vector (customizedDatasetGenerator.cpp, restated in
+    tests/golden/customized.py) at 2^20 keys: radix, bucket (plain and Dr.Top-k) and bitonic
+    against the oracle, on both sides of the boundary between the climbing prefix and the tail of
+    2^32 - 1 copies; the plain bucket path refines through four levels there."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import customized as cz
+    n = 1 << 20
+    v = cz.customized(n)
+    dev = to_dev(v)
+    T = n - 1024
+    for k in [1, 1024, T, T + 1, T + 300, n]:
+        for algo in (0, 1):
+            for alpha in (-1, 0):
+                compare(orc, dt, "customized", v, dev, k, algo, orc.FLAGS_DEFAULT, alpha=alpha)
+    for k in (1, 1 << 10, 1 << 14):
+        compare(orc, dt, "customized", v, dev, k, 2, orc.FLAGS_DEFAULT)

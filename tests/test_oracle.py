@@ -420,3 +420,34 @@ def test_verify_rows(orc):
         bad[r7[3]], bad[r7[3] + 1] = bad[r7[3] + 1], bad[r7[3]]
         codes = orc.verify_rows(base, rows, kth, okeys, bad)
         assert list(np.nonzero(codes)[0]) == [7]
+
+
+# ------------------------------------------------------------------ the reference's adversarial vector
+def test_customized_vector_closed_form(orc):
+    """customizedDatasetGenerator.cpp's vector (tests/golden/customized.py): 1024 keys climbing
+    through four ever finer levels, then copies of 2^32 - 1.  The oracle's k-th key, top-k keys
+    and indices equal the sort-based closed form on both sides of the boundary, and the plain
+    bucket select refines through one level per level of the vector (four) just past the tail."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import customized as cz
+    n = 1 << 16
+    v = cz.customized(n)
+    assert v.dtype == np.uint32 and (v[1024:] == 0xFFFFFFFF).all() and v[0] == 8388607
+    T = n - 1024
+    s = np.sort(v)
+    for k in [1, T - 1, T, T + 1, T + 10, T + 256, T + 257, T + 700, n]:
+        K = int(s[n - k])
+        gt = np.nonzero(v > K)[0]
+        eq = np.nonzero(v == K)[0][:k - gt.size]
+        want_idx = np.sort(np.concatenate([gt, eq]))
+        for algo in (0, 1):
+            for alpha in (-1, 0):
+                kk, keys, idx, st = orc.drtopk(algo, v, k, orc.params(alpha=alpha))
+                assert kk == K, (k, algo, alpha)
+                assert np.array_equal(idx, want_idx) and np.array_equal(keys, v[want_idx]), (k, algo, alpha)
+    _, _, _, st = orc.drtopk(1, v, T + 1, orc.params(alpha=0))
+    assert st["second_passes"] == 4
+    _, _, _, st = orc.drtopk(1, v, T, orc.params(alpha=0))
+    assert st["second_passes"] == 1
