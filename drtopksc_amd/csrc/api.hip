@@ -256,6 +256,15 @@ drtopk_status select_impl(const Plan& pl, const void* d_keys, void* d_workspace,
     // lane-group sampler produces one delegate per 16..64 lanes and the per-wave histogram update
     // costs it more (25-60 us at 2^30 keys) than a separate pass over the delegates (11-17 us).
     const bool fuse_first = fuse_hist && !(pl.alpha >= 6 && pl.alpha <= 8);
+    // beta path, radix, uint32 keys, 8-bit digits, beta 2, FILTER, fused first digit: the
+    // threshold is a top-digit boundary and no pass reads the delegates but the concatenation,
+    // which needs the second delegate only against it (lonely delegates below it are dropped),
+    // so the delegates are stored as the first one plus one 16-bit word holding the second's top
+    // byte and the first's offset (6 bytes per subrange, not 8 + 1).  Float keys keep full
+    // delegates: their top byte (sign and seven exponent bits) often leaves a dense threshold,
+    // which takes one digit more.
+    const bool top_form = pl.path == 2 && radix && !pl.f32 && nb == 8 && pl.beta == 2 &&
+                          (pl.flags & DRTOPK_FLAG_FILTER) && fuse_first;
     const int Lmax = bucket_max_levels(B);
     const bool timing = (pl.flags & DRTOPK_FLAG_TIMING) && h_stats;
 
@@ -272,8 +281,11 @@ drtopk_status select_impl(const Plan& pl, const void* d_keys, void* d_workspace,
     } else {
         if (pl.path == 1)
             init.first = make_sel(mode, pl.S, pl.k, nb, B, 0, 1, 1 << 20);
-        else
+        else {
             init.first = make_sel(mode, pl.D, pl.k, nb, B, 3, 0, 2);
+            // full delegates: one more digit over a dense threshold
+            if (radix && !top_form) init.first.dense_limit = 4 * pl.k;
+        }
         init.second = make_sel(mode, 0, pl.k, nb, B, 0, radix ? 0 : 1, 1 << 20);
     }
     HIPCHK(launch_init(ctrl, init, (u32*)(ws + pl.o_hist), (pl.o_zero_end - pl.o_hist) / 4, st));
@@ -335,13 +347,6 @@ drtopk_status select_impl(const Plan& pl, const void* d_keys, void* d_workspace,
         sa.fuse = radix ? (fuse_first ? 1 : 0) : 2;
         sa.dvals = dvals; sa.dpos = doff; sa.ghist = hist;
         sa.hshift = 32 - nb; sa.bins = bins; sa.ctrl = ctrl;
-        // beta path, radix, 8-bit digits, beta 2, FILTER, fused first digit: the threshold is a
-        // top-digit boundary and no pass reads the delegates but the concatenation, which needs
-        // the second delegate only against it (lonely delegates below it are dropped), so the
-        // delegates are stored as the first one plus one 16-bit word holding the second's top
-        // byte and the first's offset (6 bytes per subrange, not 8 + 1)
-        const bool top_form = pl.path == 2 && radix && nb == 8 && pl.beta == 2 &&
-                              (pl.flags & DRTOPK_FLAG_FILTER) && fuse_first && sa.staged;
         // 16-bit words after the first delegates, 16-byte aligned, inside the 8*S-byte area
         u8* dtop = top_form ? (u8*)(dvals + align_up(pl.S, 4)) : nullptr;
         sa.dtop = dtop;
@@ -353,7 +358,8 @@ drtopk_status select_impl(const Plan& pl, const void* d_keys, void* d_workspace,
         if (radix) {
             // default path: digits until the k-digit bucket holds one delegate
             // (radix_select_inplace_firsttopk :1667-1724); beta path: digits >= 3 (:1747)
-            const int ndig = pl.path == 1 ? D0 + 1 : std::max(0, D0 - 3 + 1);
+            // (beta path with full delegates: one digit more, run only when the threshold is dense)
+            const int ndig = pl.path == 1 ? D0 + 1 : (D0 >= 3 ? D0 - 3 + 1 + (top_form ? 0 : 1) : 0);
             for (int d = 0; d < ndig; ++d) {
                 if (!(d == 0 && fuse_first)) HIPCHK(launch_hist(0, dvals, fst, hist, bins, gd, st));
                 HIPCHK(launch_select(0, hist, bins, fst, ctrl, st));

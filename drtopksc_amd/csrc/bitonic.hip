@@ -52,15 +52,25 @@ __device__ __forceinline__ void lds_steps(u64* A, u32 T, u64 base, u64 len, u64 
 
 // ---------------------------------------------------------------- tile reduction (k <= TILE/2)
 // Reduces tile b of A (T <= TILE elements) to its top-k, sorted ascending, at out[b*k, b*k + k).
+// presorted: A holds runs of k each sorted ascending (a previous level's output); odd runs are
+// loaded reversed, which gives the alternating k-runs the merge needs without sorting again.
 template <u32 TILE, int NT>
-__global__ __launch_bounds__(NT) void k_bt_tile_reduce(const u64* __restrict__ A, u32 T, u32 k,
+__global__ __launch_bounds__(NT) void k_bt_tile_reduce(const u64* __restrict__ A, u32 T, u32 k, int presorted,
                                                        u64* __restrict__ out) {
     __shared__ u64 S[TILE];
     const u64 base = (u64)blockIdx.x * T;
-    for (u32 i = threadIdx.x; i < T; i += NT) S[i] = A[base + i];
-    __syncthreads();
-    // sort into alternating k-runs (tile-local directions)
-    for (u64 len = 1; len < k; len <<= 1) lds_steps<NT>(S, T, 0, len, len);
+    if (presorted) {
+        for (u32 i = threadIdx.x; i < T; i += NT) {
+            const u32 j = i & (k - 1), r0 = i - j;  // position in its run, first index of the run
+            S[i] = A[base + ((i & k) ? r0 + (k - 1 - j) : i)];
+        }
+        __syncthreads();
+    } else {
+        for (u32 i = threadIdx.x; i < T; i += NT) S[i] = A[base + i];
+        __syncthreads();
+        // sort into alternating k-runs (tile-local directions)
+        for (u64 len = 1; len < k; len <<= 1) lds_steps<NT>(S, T, 0, len, len);
+    }
     u32 cur = T;
     while (cur > k) {
         // merge: element t of the halved tile = max of the pair (i, i+k) of runs 2r, 2r+1
@@ -135,17 +145,20 @@ __global__ __launch_bounds__(WG) void k_bt_merge(const u64* __restrict__ A, u64 
 static unsigned grid_for(u64 items) { return (unsigned)std::max<u64>(1, std::min<u64>((items + WG - 1) / WG, 8192)); }
 
 // Tile reduction levels with tiles of TILE composites: n -> k.
+// Levels after the first read the sorted runs the previous one wrote.
 template <u32 TILE, int NT>
 u64* tile_levels(u64* A, u64* B, u64 n, u64 k, hipStream_t st) {
+    int presorted = 0;
     while (n > TILE) {
         hipLaunchKernelGGL((k_bt_tile_reduce<TILE, NT>), dim3((unsigned)(n / TILE)), dim3(NT), 0, st, A, TILE,
-                           (u32)k, B);
+                           (u32)k, presorted, B);
         n = (n / TILE) * k;
         std::swap(A, B);
+        presorted = 1;
     }
     // last tile: n <= TILE elements -> k (also sorts when n == k)
     const u32 T = (u32)std::max<u64>(n, 1);
-    hipLaunchKernelGGL((k_bt_tile_reduce<TILE, NT>), dim3(1), dim3(NT), 0, st, A, T, (u32)k, B);
+    hipLaunchKernelGGL((k_bt_tile_reduce<TILE, NT>), dim3(1), dim3(NT), 0, st, A, T, (u32)k, presorted, B);
     return B;
 }
 

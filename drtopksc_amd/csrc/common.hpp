@@ -63,6 +63,9 @@ struct SelState {
     u32 stop_digit;  // radix: finished once digit < stop_digit (3 on the beta path)
     u32 stop_on_one; // finished once the k-bucket holds a single key
     u32 max_passes;  // bucket: finished after this many levels (result = k-bucket lower bound)
+    u32 pad0;
+    u64 dense_limit; // radix: once digit < stop_digit, one more digit while more than this many
+                     // keys lie at or above the prefix (0: never)
 };
 
 // First 16 bytes of every workspace (single-vector and batched): the device-side error words.
@@ -205,16 +208,15 @@ __device__ __forceinline__ u32 block_excl_scan_u32(u32 v, u32* scratch, u32* tot
     return off + inc - v;
 }
 
-// Whole-wave histogram update: lanes with `active` add bin `b`.  The most common bins
-// are peeled with one LDS atomic each (two rounds), the rest go one by one.
-#ifndef DRTOPK_HIST_PEEL
-#define DRTOPK_HIST_PEEL 2
-#endif
+// Whole-wave histogram update: lanes with `active` add bin `b`.  The bins of the first PEEL
+// active lanes are peeled with one LDS atomic each (all lanes sharing that bin at once), the
+// rest go one by one; skewed keys then cost one atomic per wave, not one per lane.
+template <int PEEL = 2>
 __device__ __forceinline__ void wave_hist_add(u32* lhist, bool active, u32 b) {
     const int lane = threadIdx.x & (WAVE - 1);
     bool rem = active;
 #pragma unroll
-    for (int it = 0; it < DRTOPK_HIST_PEEL; ++it) {
+    for (int it = 0; it < PEEL; ++it) {
         const u64 act = __ballot(rem);
         if (!act) return;
         const int src = __ffsll((unsigned long long)act) - 1;

@@ -248,6 +248,22 @@ __device__ __forceinline__ void h2_flush(u32* lh, u32* ghist2, int bins2) {
     }
 }
 
+// Direct (unstaged) stores of the concatenation count their top-digit bin one key at a time.
+// Every entry is >= the threshold, so skewed keys put most of them in the threshold's own bin:
+// those are counted in a register and added once per wave at the end, the rest with LDS atomics.
+struct H2Direct {
+    u32 tbin, own = 0;
+    __device__ __forceinline__ void add(u32* lh, u32 b) {
+        if (b == tbin) ++own;
+        else atomicAdd(&lh[b], 1u);
+    }
+    // every lane of the workgroup, before h2_flush
+    __device__ __forceinline__ void flush(u32* lh) {
+        const u32 w = (u32)wave_sum_u64(own);
+        if ((threadIdx.x & (WAVE - 1)) == 0 && w) atomicAdd(&lh[tbin], w);
+    }
+};
+
 // Chunks per segment: enough segments to fill the chip (~2048), at most 16 chunks each.
 #ifndef DRTOPK_CPS_MAX
 #define DRTOPK_CPS_MAX 16
@@ -406,6 +422,7 @@ __global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64
     __shared__ u32 si[IDX ? GSTAGE : 1];
     h2_init<HIST2>(lh, bins2);
     const u32 thr = ctrl->first.result;
+    H2Direct h2d{thr >> hshift2};
     const u32 n = (u32)n64;
     const u32 Lm = (1u << alpha) - 1;
     const u32 V = (u32)(nsel << alpha);
@@ -466,7 +483,7 @@ __global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64
                     } else {
                         gk[slot] = kv[j];
                         if (IDX) gi[slot] = gid;
-                        if (HIST2) atomicAdd(&lh[kv[j] >> hshift2], 1u);
+                        if (HIST2) h2d.add(lh, kv[j] >> hshift2);
                     }
                     ++slot;
                 }
@@ -481,6 +498,7 @@ __global__ __launch_bounds__(WG) void k_gather(const u32* __restrict__ keys, u64
         }
         if (threadIdx.x == 0) cnt[s] = run;
     }
+    if (HIST2) h2d.flush(lh);
     h2_flush<HIST2>(lh, ghist2, bins2);
 }
 
@@ -630,6 +648,7 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
     h2_init<HIST2>(lh, bins2);
     const u32 n = (u32)n64;
     const u32 thr = ctrl->first.result;
+    H2Direct h2d{thr >> hshift2};
     const u32 L = 1u << alpha;
     const u32 S = (u32)((n64 + L - 1) >> alpha);
     const u32 cps = (u32)cps64;
@@ -871,7 +890,7 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
                 } else {
                     gk[slot] = o;
                     if (POS) gi[slot] = gidx;
-                    if (HIST2) atomicAdd(&lh[o >> hshift2], 1u);
+                    if (HIST2) h2d.add(lh, o >> hshift2);
                 }
             };
             if (BETA == 2 && (filter & 2)) {
@@ -982,6 +1001,7 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
             if (dd) atomicAdd((unsigned long long*)&ctrl->lonely_dropped, (unsigned long long)dd);
         }
     }
+    if (HIST2) h2d.flush(lh);
     h2_flush<HIST2>(lh, ghist2, bins2);
 }
 

@@ -29,9 +29,9 @@ struct SampleArgs {
     u32* ghist;        // fused histogram (fuse == 1)
     int hshift, bins;  // fused histogram: bin = delegate >> hshift, bins bins
     Ctrl* ctrl;
-    // beta 2, staged: when set, dvals[s] holds the first delegate and the 16-bit word s of dtop
-    // the top byte of the second (all a beta-path threshold on the top 8-bit digit needs of it)
-    // and, above it, the first's offset in the subrange (0xFF without positions); dpos unused
+    // beta 2: when set, dvals[s] holds the first delegate and the 16-bit word s of dtop the top
+    // byte of the second (all a beta-path threshold on the top 8-bit digit needs of it) and,
+    // above it, the first's offset in the subrange (0xFF without positions); dpos unused
     u8* dtop = nullptr;
 };
 hipError_t launch_sample(const SampleArgs& a, hipStream_t st);

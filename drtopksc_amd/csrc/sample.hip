@@ -462,9 +462,9 @@ __device__ __forceinline__ void staged_tile_full(const u32x4* tile, u64 k0, u32*
 
 template <class K, int BETA, int POSM, int FUSE, class LT>
 __global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ keys, u64 n, int alpha, int vec_ok,
-                                                      u64 ntiles, u32* __restrict__ dvals, void* __restrict__ dpos,
-                                                      u32* __restrict__ ghist, int hshift, int bins, Ctrl* ctrl,
-                                                      u8* __restrict__ dtop) {
+                                                      u64 t_begin, u64 ntiles, u32* __restrict__ dvals,
+                                                      void* __restrict__ dpos, u32* __restrict__ ghist, int hshift,
+                                                      int bins, Ctrl* ctrl, u8* __restrict__ dtop) {
     // one pad uint4 after every 8 uint4 spreads the threads' subrange reads over the LDS banks
     __shared__ u32x4 tile[STAGE_Q * WG + STAGE_Q * WG / 8];
     __shared__ u32 lhist[FUSE == 1 ? 256 : 1];
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ ke
     const int spt = STAGE_KEYS >> alpha;  // subranges per tile
     const int qps = alpha >= 2 ? (int)(L >> 2) : 1;  // uint4 per subrange
     u32x4 r[STAGE_Q];
-    u64 tt = blockIdx.x;
+    u64 tt = t_begin + blockIdx.x;
     if (tt < ntiles) {
         const u64 k0 = tt * STAGE_KEYS;
         stage_fetch<K>(keys, n, k0, vec_ok && k0 + STAGE_KEYS <= n, r);
@@ -543,6 +543,138 @@ __global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ ke
     fuse_flush<FUSE>(lhist, ghist, bins, mn, mx, ctrl, red);
 }
 
+// Beta 2, full tiles only, alpha a compile-time constant: each thread holds the eight 16-byte
+// loads of two future tiles in registers (64 VGPRs), so a workgroup keeps 64 KB of reads in
+// flight while it reduces the current tile from LDS; with the runtime alpha of k_sample_staged
+// every alpha's reduction shares one register allocation (130 VGPRs, 3 waves per SIMD).
+template <class K, int POSM, int FUSE, int A>
+__global__ __launch_bounds__(WG) void k_sample_stage2(const u32* __restrict__ keys, u64 nfull,
+                                                      u32* __restrict__ dvals, void* __restrict__ dpos,
+                                                      u32* __restrict__ ghist, int hshift, int bins, Ctrl* ctrl,
+                                                      u8* __restrict__ dtop) {
+    using LT = typename ListOf<2, POSM>::type;
+    __shared__ u32x4 tile[STAGE_Q * WG + STAGE_Q * WG / 8];
+    __shared__ u32 lhist[FUSE == 1 ? 256 : 1];
+    __shared__ u32 red[16];
+    fuse_init<FUSE>(lhist, bins);
+    u32 mn = 0xFFFFFFFFu, mx = 0u;
+    const u64 G = gridDim.x;
+    u32x4 ra[STAGE_Q], rb[STAGE_Q];
+    auto fetch = [&](u64 t, u32x4 (&r)[STAGE_Q]) {
+        if (t < nfull) {
+            const u32x4* p = (const u32x4*)(keys + t * STAGE_KEYS);
+#pragma unroll
+            for (int u = 0; u < STAGE_Q; ++u) r[u] = load4(p + u * WG + threadIdx.x);
+        }
+    };
+    auto process = [&](u64 t, u32x4 (&r)[STAGE_Q], u64 tn) {
+#pragma unroll
+        for (int u = 0; u < STAGE_Q; ++u) {
+            const int q = u * WG + threadIdx.x;
+            u32x4 v = r[u];
+            v.x = K::to_ord(v.x); v.y = K::to_ord(v.y); v.z = K::to_ord(v.z); v.w = K::to_ord(v.w);
+            tile[q + (q >> 3)] = v;
+        }
+        fetch(tn, r);
+        __syncthreads();
+        staged_tile_full<A, 2, POSM, FUSE, LT>(tile, t * STAGE_KEYS, dvals, dpos, dtop, lhist, hshift, &mn, &mx);
+        __syncthreads();
+    };
+    fetch(blockIdx.x, ra);
+    fetch(blockIdx.x + G, rb);
+    for (u64 t = blockIdx.x; t < nfull; t += 2 * G) {  // uniform per workgroup
+        process(t, ra, t + 2 * G);
+        if (t + G >= nfull) break;
+        process(t + G, rb, t + 3 * G);
+    }
+    fuse_flush<FUSE>(lhist, ghist, bins, mn, mx, ctrl, red);
+}
+
+// ---------------------------------------------------------------- beta 2, lane groups with DPP
+// Beta 2, full 8192-key chunks, alpha 2..5 a compile-time constant, no LDS staging: lane l of a
+// wave holds keys 4l..4l+3 of a 256-key wave tile (eight 16-byte loads in flight per lane), takes
+// their top two (with the lowest index of the maximum) and merges them across the 2^(A-2) lanes
+// of its subrange with DPP lane moves: quad_perm swaps within a quad (xor 1, xor 2) and the row
+// half mirror (lane i <-> 7 - i of each 8) for 8-lane groups.  Every lane of a group ends with
+// the subrange's result; the group's first lane stores it.
+__device__ __forceinline__ u32 dpp_x1(u32 v) { return (u32)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false); }
+__device__ __forceinline__ u32 dpp_x2(u32 v) { return (u32)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false); }
+__device__ __forceinline__ u32 dpp_hm(u32 v) { return (u32)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false); }
+
+template <int STEP>
+__device__ __forceinline__ u32 dpp_partner(u32 v) {
+    if constexpr (STEP == 0) return dpp_x1(v);
+    else if constexpr (STEP == 1) return dpp_x2(v);
+    else return dpp_hm(v);
+}
+
+// top two (multiset) of two lanes' top twos, and the lowest index holding the maximum
+template <int STEP, bool P>
+__device__ __forceinline__ void top2_merge(u32& m1, u32& m2, u32& p1) {
+    const u32 b1 = dpp_partner<STEP>(m1), b2 = dpp_partner<STEP>(m2);
+    m2 = max(min(m1, b1), max(m2, b2));
+    if (P) {
+        const u32 bp = dpp_partner<STEP>(p1);
+        p1 = m1 > b1 ? p1 : (b1 > m1 ? bp : min(p1, bp));
+    }
+    m1 = max(m1, b1);
+}
+
+template <class K, int POSM, int FUSE, int A>
+__global__ __launch_bounds__(WG) void k_sample_dpp2(const u32* __restrict__ keys, u64 nchunks, u32* __restrict__ dvals,
+                                                    void* __restrict__ dpos, u32* __restrict__ ghist, int hshift,
+                                                    int bins, Ctrl* ctrl, u8* __restrict__ dtop) {
+    constexpr int G = 1 << (A - 2);  // lanes per subrange
+    constexpr bool P = POSM == 2;
+    __shared__ u32 lhist[FUSE == 1 ? 256 : 1];
+    __shared__ u32 red[16];
+    fuse_init<FUSE>(lhist, bins);
+    u32 mn = 0xFFFFFFFFu, mx = 0u;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const bool leader = (lane & (G - 1)) == 0;
+    for (u64 chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const u64 c0 = chunk * GROUP_CHUNK;
+        const u32x4* base4 = (const u32x4*)(keys + c0);
+        u32x4 raw[GROUP_BATCHES];
+#pragma unroll
+        for (int b = 0; b < GROUP_BATCHES; ++b) raw[b] = load4(base4 + b * WG + threadIdx.x);
+#pragma unroll
+        for (int b = 0; b < GROUP_BATCHES; ++b) {
+            const u32 i0 = (u32)(c0 + 4ull * (b * WG + threadIdx.x));  // n < 2^31
+            const u32 x0 = K::to_ord(raw[b].x), x1 = K::to_ord(raw[b].y), x2 = K::to_ord(raw[b].z),
+                      x3 = K::to_ord(raw[b].w);
+            // top two of the lane's four keys, index of the first maximum
+            u32 m1 = x0, m2 = 0, p1 = i0;
+            m2 = max(m2, min(m1, x1)); p1 = x1 > m1 ? i0 + 1 : p1; m1 = max(m1, x1);
+            m2 = max(m2, min(m1, x2)); p1 = x2 > m1 ? i0 + 2 : p1; m1 = max(m1, x2);
+            m2 = max(m2, min(m1, x3)); p1 = x3 > m1 ? i0 + 3 : p1; m1 = max(m1, x3);
+            if constexpr (G >= 2) top2_merge<0, P>(m1, m2, p1);
+            if constexpr (G >= 4) top2_merge<1, P>(m1, m2, p1);
+            if constexpr (G >= 8) top2_merge<2, P>(m1, m2, p1);
+            const u32 s = i0 >> A;
+            if (leader) {
+                const u32 off = p1 - (s << A);
+                if (dtop) {
+                    dvals[s] = m1;
+                    ((uint16_t*)dtop)[s] = (uint16_t)((m2 >> 24) | ((P ? off : 0xFFu) << 8));
+                } else {
+                    *(uint2*)(dvals + 2 * (u64)s) = make_uint2(m1, m2);
+                    if (P) ((uint8_t*)dpos)[s] = (uint8_t)off;
+                }
+            }
+            if (FUSE == 1) {
+                wave_hist_add(lhist, leader, m1 >> hshift);
+                wave_hist_add(lhist, leader, m2 >> hshift);
+            }
+            if (FUSE == 2 && leader) {
+                mn = min(mn, m2);
+                mx = max(mx, m1);
+            }
+        }
+    }
+    fuse_flush<FUSE>(lhist, ghist, bins, mn, mx, ctrl, red);
+}
+
 // ---------------------------------------------------------------- dispatch
 constexpr u64 SAMPLE_GRID = 8192;  // persistent workgroups (32 per CU)
 
@@ -552,13 +684,56 @@ static hipError_t launch_sample_t(const SampleArgs& a, hipStream_t st) {
     const u64 L = 1ull << a.alpha;
     const u64 S = (a.n + L - 1) >> a.alpha;
     const int vec_ok = ((uintptr_t)a.keys & 15) == 0;
-    if (a.staged && a.alpha <= 5) {
+    // the compact beta-2 form (dtop) is written by the staged and the DPP lane-group kernels
+    const bool dpp2_ok = BETA == 2 && POSM != 1 && FUSE != 0 && a.alpha >= 2 && a.alpha <= 5 && vec_ok &&
+                         a.n >= (u64)GROUP_CHUNK;
+    if ((a.staged && a.alpha <= 5) || (a.dtop && !dpp2_ok)) {
         // exactly the resident workgroups (LDS allows 4 per CU): every one keeps its pipeline full
         const u64 ntiles = (a.n + STAGE_KEYS - 1) / STAGE_KEYS;
-        const unsigned grid = (unsigned)std::min<u64>(ntiles, 4ull * device_cus());
-        hipLaunchKernelGGL((k_sample_staged<K, BETA, POSM, FUSE, LT>), dim3(grid), dim3(WG), 0, st, a.keys, a.n,
-                           a.alpha, vec_ok, ntiles, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl,
-                           BETA == 2 ? a.dtop : nullptr);
+        u64 t0 = 0;  // tiles before t0 are done by the beta-2 kernel
+        if constexpr (BETA == 2 && POSM != 1 && FUSE != 0) {
+            const u64 nfull = vec_ok ? a.n / STAGE_KEYS : 0;
+            if (a.alpha >= 2 && nfull > 0) {
+                const unsigned g2 = (unsigned)std::min<u64>(nfull, 4ull * device_cus());
+#define S2_LAUNCH(AA)                                                                                         \
+    hipLaunchKernelGGL((k_sample_stage2<K, POSM, FUSE, AA>), dim3(g2), dim3(WG), 0, st, a.keys, nfull, a.dvals, \
+                       a.dpos, a.ghist, a.hshift, a.bins, a.ctrl, a.dtop)
+                switch (a.alpha) {
+                    case 2: S2_LAUNCH(2); break;
+                    case 3: S2_LAUNCH(3); break;
+                    case 4: S2_LAUNCH(4); break;
+                    default: S2_LAUNCH(5); break;
+                }
+#undef S2_LAUNCH
+                t0 = nfull;
+            }
+        }
+        if (t0 < ntiles) {
+            const unsigned grid = (unsigned)std::min<u64>(ntiles - t0, 4ull * device_cus());
+            hipLaunchKernelGGL((k_sample_staged<K, BETA, POSM, FUSE, LT>), dim3(grid), dim3(WG), 0, st, a.keys, a.n,
+                               a.alpha, vec_ok, t0, ntiles, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl,
+                               BETA == 2 ? a.dtop : nullptr);
+        }
+    } else if (dpp2_ok) {
+        // lane groups with DPP merges over the full chunks, the staged kernel over the tail
+        const u64 nfull = a.n / GROUP_CHUNK;
+        const unsigned g2 = (unsigned)std::min<u64>(nfull, SAMPLE_GRID);
+#define D2_LAUNCH(AA)                                                                                        \
+    hipLaunchKernelGGL((k_sample_dpp2<K, POSM, FUSE, AA>), dim3(g2), dim3(WG), 0, st, a.keys, nfull, a.dvals, \
+                       a.dpos, a.ghist, a.hshift, a.bins, a.ctrl, a.dtop)
+        switch (a.alpha) {
+            case 2: D2_LAUNCH(2); break;
+            case 3: D2_LAUNCH(3); break;
+            case 4: D2_LAUNCH(4); break;
+            default: D2_LAUNCH(5); break;
+        }
+#undef D2_LAUNCH
+        static_assert(GROUP_CHUNK == STAGE_KEYS, "the tail is resumed in staged tiles of the same size");
+        const u64 ntiles = (a.n + STAGE_KEYS - 1) / STAGE_KEYS;
+        if (nfull < ntiles)
+            hipLaunchKernelGGL((k_sample_staged<K, BETA, POSM, FUSE, LT>), dim3(1), dim3(WG), 0, st, a.keys, a.n,
+                               a.alpha, vec_ok, nfull, ntiles, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl,
+                               BETA == 2 ? a.dtop : nullptr);
     } else if (a.alpha <= 8) {
         const u64 nchunks = (a.n + GROUP_CHUNK - 1) / GROUP_CHUNK;
         const unsigned grid = (unsigned)std::min<u64>(nchunks, SAMPLE_GRID);
@@ -603,7 +778,7 @@ static hipError_t launch_sample_k(const SampleArgs& a, hipStream_t st) {
 
 hipError_t launch_sample(const SampleArgs& a, hipStream_t st) {
     if (a.posmode == 2 && a.beta < 2) return hipErrorInvalidValue;
-    if (a.dtop && !(a.beta == 2 && a.staged && a.alpha <= 5)) return hipErrorInvalidValue;
+    if (a.dtop && !(a.beta == 2 && a.alpha <= 5)) return hipErrorInvalidValue;
     return a.f32 ? launch_sample_k<KeyF32>(a, st) : launch_sample_k<KeyU32>(a, st);
 }
 

@@ -132,12 +132,21 @@ __global__ __launch_bounds__(WG) void k_hist(const u32* __restrict__ arr, Seg se
     }
     u32* h = GLOBALH ? ghist : lh;
     const u32 dmask = (u32)bins - 1;
+    // LDS histograms: one peel round, so a wave of equal digits (skewed or tied keys) costs
+    // one atomic; the global-memory histogram of 16-bit digits counts key by key
     auto count = [&](u32 raw) {
         const u32 v = RAW ? K::to_ord(raw) : raw;
         if (MODE == 0) {
-            if ((v & mask) == flag) atomicAdd(&h[(v >> shright) & dmask], 1u);
+            const bool hit = (v & mask) == flag;
+            const u32 b = (v >> shright) & dmask;
+            if (GLOBALH) {
+                if (hit) atomicAdd(&h[b], 1u);
+            } else {
+                wave_hist_add<1>(h, hit, b);
+            }
         } else {
-            if (v >= lo && v - lo <= W) atomicAdd(&h[bucket_of(v, lo, W, Bm1, inv)], 1u);
+            const bool hit = v >= lo && v - lo <= W;
+            wave_hist_add<1>(h, hit, hit ? bucket_of(v, lo, W, Bm1, inv) : 0u);
         }
     };
     if (SEG) for_seg(seg, count);
@@ -291,7 +300,12 @@ __global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState
         const u32 flag = sel->flag | (i << (d * (int)sel->nb));
         sel->flag = flag;
         sel->digit = d - 1;
-        if (d - 1 < (int)sel->stop_digit || (sel->stop_on_one && found[2] == 1)) {
+        bool fin = d - 1 < (int)sel->stop_digit;
+        // a dense threshold (more than dense_limit keys at or above it: float keys, whose top
+        // byte is the sign and seven exponent bits, or skewed keys) takes one digit more
+        if (fin && sel->dense_limit && d - 1 == (int)sel->stop_digit - 1 && m - (below + found[1]) > sel->dense_limit)
+            fin = false;
+        if (fin || (sel->stop_on_one && found[2] == 1)) {
             sel->done = 1;
             sel->result = flag;
         }

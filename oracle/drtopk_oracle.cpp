@@ -371,7 +371,9 @@ static void bitonic_topk(std::vector<u64>& A, u64 k) {
 // --- the pipeline ---
 // sample_radix_select (radixselect.cuh:1783-1894), sample_bucket_select
 // (bucketselect_combined.cuh:1280-1358), bitonic driver (sampleBitonic.cu:283-308).
-extern "C" int orc_drtopk(int algo, const u32* keys, u64 n, u64 k, const drtopk_params* p, u32* kth_out,
+// f32: the ordered keys come from float32 keys (only the storage of the beta path's delegates,
+// and so the dense-threshold rule below, depends on it).
+extern "C" int orc_drtopk(int algo, const u32* keys, u64 n, u64 k, const drtopk_params* p, int f32, u32* kth_out,
                           u32* out_keys, i64* out_idx, drtopk_stats* st) {
     int r[6];
     if (orc_resolve(algo, n, k, p, r)) return 1;
@@ -464,6 +466,17 @@ extern "C" int orc_drtopk(int algo, const u32* keys, u64 n, u64 k, const drtopk_
                 // radix_select_inplace_firsttopk_beta (radixselect.cuh:1728-1780): digits while CurrentDigit>=3.
                 RadixState rs{0, 0, 32 / nb - 1, 0};
                 while (rs.digit >= 3) { radix_digit_step(dv.data(), D, k, nb, rs); ++fpasses; }
+                // Build deviation: one digit more when the threshold is dense, i.e. more than 4k
+                // delegates lie at or above it (float keys, whose top byte is the sign and seven
+                // exponent bits; skewed keys).  Any prefix of the k-th largest delegate is a valid
+                // threshold, so the outputs are unchanged; the concatenation shrinks.  Not with the
+                // build's compact delegates (uint32 keys, 8-bit digits, beta 2, FILTER), which
+                // keep only the second delegate's top byte and so the reference's one digit.
+                const bool compact = !f32 && nb == 8 && beta == 2 && (flags & DRTOPK_FLAG_FILTER);
+                if (!compact && fpasses > 0 && rs.digit == 2 && D - rs.below > 4 * k) {
+                    radix_digit_step(dv.data(), D, k, nb, rs);
+                    ++fpasses;
+                }
                 thr = rs.flag;
             } else {
                 // bucket_select_inplace_betaFirstTopk (bucketselect_combined.cuh:973-1095): min/max, level 0,

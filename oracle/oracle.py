@@ -79,7 +79,7 @@ def lib():
                                   ctypes.POINTER(ctypes.c_int)]
         L.orc_sample.argtypes = [u32p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, u32p, u32p]
         L.orc_drtopk.argtypes = [ctypes.c_int, u32p, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(Params),
-                                 u32p, u32p, i64p, ctypes.POINTER(Stats)]
+                                 ctypes.c_int, u32p, u32p, i64p, ctypes.POINTER(Stats)]
         L.orc_bitonic_topk64.argtypes = [u64p, ctypes.c_uint64, ctypes.c_uint64, u64p]
         L.orc_radix_kth.argtypes = [u32p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
         L.orc_radix_kth.restype = ctypes.c_uint32
@@ -165,14 +165,15 @@ def sample(okeys: np.ndarray, alpha: int, beta: int):
     return vals, pos
 
 
-def drtopk(algo: int, okeys: np.ndarray, k: int, p: Params):
-    """Restated Dr.Top-k: returns (kth, topk_keys, topk_idx, stats dict)."""
+def drtopk(algo: int, okeys: np.ndarray, k: int, p: Params, f32: bool = False):
+    """Restated Dr.Top-k: returns (kth, topk_keys, topk_idx, stats dict).  f32: the ordered keys
+    come from float32 keys (the beta path then keeps full delegates)."""
     okeys = np.ascontiguousarray(okeys, dtype=np.uint32)
     kth = ctypes.c_uint32()
     keys = np.empty(k, np.uint32)
     idx = np.empty(k, np.int64)
     st = Stats()
-    rc = lib().orc_drtopk(algo, _p(okeys), okeys.size, k, ctypes.byref(p), ctypes.byref(kth), _p(keys),
+    rc = lib().orc_drtopk(algo, _p(okeys), okeys.size, k, ctypes.byref(p), int(f32), ctypes.byref(kth), _p(keys),
                           _p(idx, ctypes.c_int64), ctypes.byref(st))
     if rc == 1:
         raise ValueError("invalid parameters")

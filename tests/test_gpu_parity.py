@@ -48,7 +48,7 @@ def compare(orc, dt, name, okeys, dev, k, algo, flags, alpha=-1, beta=2, nb=8, B
     po = orc.params(alpha=alpha, beta=beta, digit_bits=nb, num_buckets=B, flags=flags)
     # every listed case must be one the reference's parameter rules accept
     assert orc.resolve(algo, okeys.size, k, po) is not None, (name, "parameters rejected", k, alpha, beta)
-    K, keys, idx, ost = orc.drtopk(algo, okeys, k, po)
+    K, keys, idx, ost = orc.drtopk(algo, okeys, k, po, f32=f32)
     pg = dt.params(alpha=alpha, beta=beta, digit_bits=nb, num_buckets=B, flags=flags)
     gk, gkeys, gidx, gst = run_gpu(dt, dev, k, algo, pg)
     gk_bits = gk.view(np.uint32)[0]
@@ -130,7 +130,7 @@ def test_output_subsets(orc, dt, want):
                 continue
             print(f"[case] outputs={want} {name} k={k}", flush=True)
             po = orc.params(flags=orc.FLAGS_DEFAULT)
-            K, keys, idx, ost = orc.drtopk(0, ok, k, po)
+            K, keys, idx, ost = orc.drtopk(0, ok, k, po, f32=f32)
             kth, tk, ti, gst = dt.select(dev, k, p=dt.params(), want_keys=(want == "keys"),
                                          want_idx=(want == "idx"), stats=True)
             gk = kth.cpu().numpy().view(np.uint32)[:1]

@@ -284,6 +284,11 @@ def check_invariants(orc, algo, v, k, st, flags):
         if algo == 0:
             sh = 3 * nb  # digits 32/nb-1 .. 3 are resolved, the lower three digits stay 0
             want = (dk >> sh) << sh if 32 // nb - 1 >= 3 else 0
+            # with full delegates (not uint32 keys with 8-bit digits, beta 2 and FILTER), a dense
+            # threshold (more than 4k delegates at or above it) takes one digit more
+            compact = nb == 8 and beta == 2 and (flags & 1)
+            if not compact and 32 // nb - 1 >= 3 and int(np.sum(real.astype(np.uint64) >= want)) > 4 * k:
+                want = (dk >> (2 * nb)) << (2 * nb)
             assert thr == want
         # every key >= threshold reaches the concatenation (filter) or its whole subrange does
         n_ge = int(np.sum(v >= thr))
@@ -451,3 +456,22 @@ def test_customized_vector_closed_form(orc):
     assert st["second_passes"] == 4
     _, _, _, st = orc.drtopk(1, v, T, orc.params(alpha=0))
     assert st["second_passes"] == 1
+
+
+def test_dense_beta_threshold_takes_one_more_digit(orc):
+    """Float keys U(-1,1): the top byte of their ordered bits is the sign and seven exponent bits,
+    so the reference's one-digit beta-path threshold leaves far more than 4k delegates above it.
+    With full delegates (float keys) the threshold takes the second digit of the k-th largest
+    delegate; the compact uint32 form keeps the reference's one digit.  Outputs are identical."""
+    n, k = 1 << 20, 1 << 13
+    v = orc.f32_to_ordered(orc.generate(4, 77, n))
+    p = orc.params()
+    K2, keys2, idx2 = np_topk(v, k)
+    vals, _ = np_sample(v, 5, 2)  # alpha 5 = (20 + 3 - 13) // 2, beta 2
+    dk = int(np.sort(vals)[vals.size - k])
+    assert int(np.sum(vals >= ((dk >> 24) << 24))) > 4 * k  # the one-byte threshold is dense here
+    for f32, want, passes in ((True, (dk >> 16) << 16, 2), (False, (dk >> 24) << 24, 1)):
+        K, keys, idx, st = orc.drtopk(0, v, k, p, f32=f32)
+        assert st["alpha"] == 5 and st["path"] == 2
+        assert st["threshold"] == want and st["first_passes"] == passes, (f32, hex(st["threshold"]))
+        assert K == K2 and np.array_equal(idx, idx2) and np.array_equal(keys, keys2)
