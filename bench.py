@@ -184,6 +184,11 @@ def configs_leg(dt, torch, oracle, R: Runner, v_uni: np.ndarray, dev_uni, P: int
     out = {}
     t0 = time.time()
     kfew = [1, 1 << 10, 1 << 16, 1 << 20, 1 << 22, 1 << 24]
+    # the headline sweep with the measured (alpha, beta) table instead of the reference policy
+    rows = R.sweep(dev_uni, ks, p=dt.params(alpha=dt.ALPHA_TUNED))
+    tot = sum(r["ms"] for r in rows)
+    out["uniform_2p30_tuned"] = {"alpha": "measured table (DRTOPK_ALPHA_TUNED)", "sweep_ms": round(tot, 4),
+                                 "gelem_s": round(len(ks) * dev_uni.numel() / (tot * 1e-3) / 1e9, 1), "rows": rows}
     # bucket family on the headline vector
     out["uniform_2p30_bucket"] = {"algo": "bucket", "rows": R.sweep(dev_uni, ks, algo=dt.ALGO_BUCKET,
                                                                     ref=REF_BUCKET_UNIFORM_MS)}

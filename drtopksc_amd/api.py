@@ -24,6 +24,7 @@ ALGO_RADIX, ALGO_BUCKET, ALGO_BITONIC = 0, 1, 2
 U32, F32 = 0, 1
 FLAG_FILTER, FLAG_BETA, FLAG_SHUFFLE, FLAG_TIMING = 0x1, 0x2, 0x4, 0x100
 FLAGS_DEFAULT = FLAG_FILTER | FLAG_BETA | FLAG_SHUFFLE
+ALPHA_AUTO, ALPHA_TUNED = -1, -2  # reference policy; measured table (radix family)
 
 _STATUS = {0: "OK", 1: "INVALID_ARG", 2: "WORKSPACE_TOO_SMALL", 3: "UNSUPPORTED", 4: "HIP", 5: "INTERNAL",
            6: "BUDGET"}

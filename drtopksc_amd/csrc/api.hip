@@ -73,6 +73,20 @@ struct Plan {
     u64 npad1, npad2, o_bufA, o_bufB, o_dsel;  // bitonic family
 };
 
+#include "single_tuning.inc"
+
+// The measured (alpha, beta) of the radix family for (floor log2 n, ceil log2 k), if any.
+bool single_tuned(u64 n, u64 k, int* alpha, int* beta) {
+    const int ln = floor_log2(n), lk = ceil_log2(k);
+    for (const auto& c : kSingleTuned)
+        if (c[2] != 0 && c[0] == ln && c[1] == lk) {
+            *alpha = c[2];
+            *beta = c[3];
+            return true;
+        }
+    return false;
+}
+
 // Parameter policy: radix alpha = floor(0.5*(log2|V| - log2 k + 3)), beta path iff alpha<=5
 // (baseline+filter+beta+shuffle/main.cu:83-89); bucket: alpha++ when alpha<5
 // (bucket_select/main.cu:66-67); bitonic: user alpha, auto = radix formula in [5,16].
@@ -88,13 +102,25 @@ drtopk_status resolve(drtopk_algo algo, drtopk_dtype dtype, u64 n, u64 k, const 
     if (nb != 1 && nb != 2 && nb != 4 && nb != 8 && nb != 16) return fail(DRTOPK_ERR_INVALID_ARG, "digit_bits %d not in {1,2,4,8,16}", nb);
     if (p->num_buckets < 4 || p->num_buckets > 4096) return fail(DRTOPK_ERR_INVALID_ARG, "num_buckets %d not in [4, 4096]", p->num_buckets);
     if (p->beta < 1 || p->beta > 4) return fail(DRTOPK_ERR_INVALID_ARG, "beta %d not in [1, 4]", p->beta);
-    if (p->alpha < -1 || p->alpha > 30) return fail(DRTOPK_ERR_INVALID_ARG, "alpha %d not in [-1, 30]", p->alpha);
+    if (p->alpha < -2 || p->alpha > 30) return fail(DRTOPK_ERR_INVALID_ARG, "alpha %d not in [-2, 30]", p->alpha);
     if (algo == DRTOPK_ALGO_BITONIC && (k & (k - 1)))
         return fail(DRTOPK_ERR_INVALID_ARG, "bitonic top-k needs a power-of-two k (got %llu)", (unsigned long long)k);
     const int Ln = floor_log2(n);
     int alpha = p->alpha;
-    const bool autoa = alpha < 0;
-    if (autoa) {
+    // measured choice: its beta replaces p->beta, and beta 1 means one delegate per subrange
+    int tuned_beta = 0;
+    if (alpha == DRTOPK_ALPHA_TUNED) {
+        int ta = 0, tb = 0;
+        if (algo == DRTOPK_ALGO_RADIX && single_tuned(n, k, &ta, &tb)) {
+            alpha = ta;
+            tuned_beta = tb;
+        } else {
+            alpha = DRTOPK_ALPHA_AUTO;
+        }
+    }
+    const bool autoa = alpha < 0 || tuned_beta > 0;  // a tuned alpha may be lowered like the policy's
+    if (tuned_beta > 0 && alpha > Ln) alpha = Ln;
+    if (alpha < 0) {
         alpha = (Ln + 3 - ceil_log2(k)) / 2;
         if (alpha < 0) alpha = 0;
         if (algo == DRTOPK_ALGO_BUCKET && alpha < 5) alpha++;
@@ -102,14 +128,16 @@ drtopk_status resolve(drtopk_algo algo, drtopk_dtype dtype, u64 n, u64 k, const 
         if (alpha > Ln) alpha = Ln;
     }
     if (alpha > Ln) return fail(DRTOPK_ERR_INVALID_ARG, "alpha %d larger than log2(n) = %d", alpha, Ln);
+    const int beta_in = tuned_beta > 0 ? tuned_beta : p->beta;
+    const bool beta_flag = tuned_beta > 0 ? tuned_beta >= 2 : (p->flags & DRTOPK_FLAG_BETA) != 0;
     auto want_beta = [&](int a) {
-        return algo != DRTOPK_ALGO_BITONIC && (p->flags & DRTOPK_FLAG_BETA) && a <= 5 && p->beta >= 2;
+        return algo != DRTOPK_ALGO_BITONIC && beta_flag && a <= 5 && beta_in >= 2;
     };
     auto valid = [&](int a) -> bool {
         if (a == 0) return true;
         u64 L = 1ull << a, S = cdiv(n, L);
         if (algo == DRTOPK_ALGO_BITONIC) return S > k;
-        if (want_beta(a)) return p->beta <= (int)L && real_delegates(n, a, p->beta) >= k;
+        if (want_beta(a)) return beta_in <= (int)L && real_delegates(n, a, beta_in) >= k;
         return S >= k;
     };
     if (!valid(alpha)) {
@@ -119,7 +147,7 @@ drtopk_status resolve(drtopk_algo algo, drtopk_dtype dtype, u64 n, u64 k, const 
         while (alpha > 0 && !valid(alpha)) --alpha;
     }
     int beta = 1, path = alpha == 0 ? 0 : 1;
-    if (alpha > 0 && want_beta(alpha)) { beta = p->beta; path = 2; }
+    if (alpha > 0 && want_beta(alpha)) { beta = beta_in; path = 2; }
     pl->algo = algo;
     pl->f32 = dtype == DRTOPK_F32;
     pl->n = n; pl->k = k;

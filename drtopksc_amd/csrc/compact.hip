@@ -95,10 +95,7 @@ __device__ __forceinline__ void count_gt_eq(const u32 (&v)[N], u32 K0, bool whol
 // chunk's run out with coalesced stores (a thread's own survivors sit at scattered global
 // positions, so storing them directly costs one cache line per lane per store).
 constexpr u32 STAGE = 4096;
-#ifndef DRTOPK_G_STAGE
-#define DRTOPK_G_STAGE 4096
-#endif
-constexpr u32 GSTAGE = DRTOPK_G_STAGE;  // survivors of a default-path gather chunk staged in LDS
+constexpr u32 GSTAGE = 4096;  // survivors of a default-path gather chunk staged in LDS
 
 template <bool IDX, bool HIST2>
 __device__ __forceinline__ void flush_stage(const u32* sk, const u32* si, u32 cnt, u32* __restrict__ gk,
@@ -265,10 +262,8 @@ struct H2Direct {
 };
 
 // Chunks per segment: enough segments to fill the chip (~2048), at most 16 chunks each.
-#ifndef DRTOPK_CPS_MAX
-#define DRTOPK_CPS_MAX 16
-#endif
-u64 chunks_per_segment(u64 nchunks) { return std::max<u64>(1, std::min<u64>(DRTOPK_CPS_MAX, nchunks / 2048)); }
+constexpr u64 CPS_MAX = 16;
+u64 chunks_per_segment(u64 nchunks) { return std::max<u64>(1, std::min<u64>(CPS_MAX, nchunks / 2048)); }
 
 // ---------------------------------------------------------------- default-path concatenation
 // alpha == 1: one subrange id per key pair; the plain loop below.
@@ -541,7 +536,7 @@ u64 gather_nseg(u64 V) {
 }
 
 // ---------------------------------------------------------------- beta-path range selection + concatenation
-constexpr int BSPT = DRTOPK_BG_BSPT;  // subranges per thread of the beta gather
+constexpr int BSPT = BETA_SUBS_PER_THREAD;  // subranges per thread of the beta gather
 
 // Per chunk of 1024 subranges: (1) each thread classifies its 4 subranges from their
 // delegates and packed offsets and the selected ones are listed in LDS in id order;
@@ -550,16 +545,10 @@ constexpr int BSPT = DRTOPK_BG_BSPT;  // subranges per thread of the beta gather
 // per subrange; (3) a block scan places every subrange's output (selected: its kept keys,
 // lonely: its delegates, in index order); (4) each listed key goes to its slot by a prefix
 // sum over its subrange's lanes.  Survivors are staged in LDS and leave with coalesced stores.
-#ifndef DRTOPK_BG_STAGE
-#define DRTOPK_BG_STAGE 1024  // with 5 waves per SIMD below: 2-4% faster than 2048 at 4 waves
-#endif
-constexpr u32 BSTAGE = DRTOPK_BG_STAGE;  // staged survivors per chunk (more: direct stores)
-#ifndef DRTOPK_BG_ONEROUND
-#define DRTOPK_BG_ONEROUND 1
-#endif
-#ifndef DRTOPK_BG_KBUF
-#define DRTOPK_BG_KBUF 1024  // keys of listed subranges held in LDS at 5 waves per SIMD (2048, 4096: slower)
-#endif
+constexpr u32 BSTAGE = 1024;  // staged survivors per chunk (more: direct stores); at 5 waves per SIMD
+                              // 2-4% faster than 2048 at 4 waves
+constexpr u32 BKBUF = 1024;   // keys of listed subranges held in LDS at 5 waves per SIMD (2048,
+                              // 4096: slower)
 
 template <int BETA, bool WHOLE = false>
 __device__ __forceinline__ void load_deleg(const u32* __restrict__ dvals, u64 s0, u64 S, u32 (&d)[BSPT][BETA]) {
@@ -1028,7 +1017,7 @@ static u64 beta_slots_t() {
     return (u64)nb * device_cus();
 }
 static u64 beta_slots(int alpha) {
-    return beta_waves(alpha) == 8 ? beta_slots_t<8, 1024>() : beta_slots_t<5, DRTOPK_BG_KBUF>();
+    return beta_waves(alpha) == 8 ? beta_slots_t<8, 1024>() : beta_slots_t<5, BKBUF>();
 }
 
 // Chunks per beta-gather segment.  A workgroup runs one segment at a time, so equal segments
@@ -1037,12 +1026,8 @@ static u64 beta_slots(int alpha) {
 // alpha 4: 4096 segments would be 2.7 rounds of the 1536 resident workgroups).
 static u64 beta_cps(u64 nch, int alpha) {
     const u64 cps = chunks_per_segment(nch);
-#if DRTOPK_BG_ONEROUND
     const u64 slots = beta_slots(alpha);
     if (nch > slots * cps) return (nch + slots - 1) / slots;
-#else
-    (void)alpha;
-#endif
     return cps;
 }
 u64 beta_nseg(u64 S, int alpha) {
@@ -1075,7 +1060,7 @@ static hipError_t launch_beta_t(const BetaGatherArgs& a, hipStream_t st) {
         else hipLaunchKernelGGL((k_beta_gather<K, BETA, false, false, W, KB>), dim3(grid), dim3(WG), 0, st, B_ARGS);          \
     } while (0)
     if (beta_waves(a.alpha) == 8) B_LAUNCH(8, 1024);
-    else B_LAUNCH(5, DRTOPK_BG_KBUF);
+    else B_LAUNCH(5, BKBUF);
 #undef B_LAUNCH
 #undef B_ARGS
     return hipGetLastError();

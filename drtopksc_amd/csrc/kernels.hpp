@@ -115,10 +115,8 @@ struct BetaGatherArgs {
     const u8* dtop = nullptr;  // beta 2: delegates in the sampler's dtop form (dvals[s] first, 16-bit dtop words)
     u64 seg_limit = 0;         // entries a segment may hold (0: its capacity)
 };
-#ifndef DRTOPK_BG_BSPT
-#define DRTOPK_BG_BSPT 4  // 4 measured faster than 8 (fewer VGPRs, more resident workgroups)
-#endif
-constexpr u64 BETA_TILE_SUBS = 256 * DRTOPK_BG_BSPT;  // subranges per chunk of the beta gather
+constexpr int BETA_SUBS_PER_THREAD = 4;  // 4 measured faster than 8 (fewer VGPRs, more resident workgroups)
+constexpr u64 BETA_TILE_SUBS = 256 * BETA_SUBS_PER_THREAD;  // subranges per chunk of the beta gather
 hipError_t launch_beta_gather(const BetaGatherArgs& a, hipStream_t st);
 u64 beta_segcap(u64 S, int alpha);
 u64 beta_nseg(u64 S, int alpha);

@@ -543,53 +543,6 @@ __global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ ke
     fuse_flush<FUSE>(lhist, ghist, bins, mn, mx, ctrl, red);
 }
 
-// Beta 2, full tiles only, alpha a compile-time constant: each thread holds the eight 16-byte
-// loads of two future tiles in registers (64 VGPRs), so a workgroup keeps 64 KB of reads in
-// flight while it reduces the current tile from LDS; with the runtime alpha of k_sample_staged
-// every alpha's reduction shares one register allocation (130 VGPRs, 3 waves per SIMD).
-template <class K, int POSM, int FUSE, int A>
-__global__ __launch_bounds__(WG) void k_sample_stage2(const u32* __restrict__ keys, u64 nfull,
-                                                      u32* __restrict__ dvals, void* __restrict__ dpos,
-                                                      u32* __restrict__ ghist, int hshift, int bins, Ctrl* ctrl,
-                                                      u8* __restrict__ dtop) {
-    using LT = typename ListOf<2, POSM>::type;
-    __shared__ u32x4 tile[STAGE_Q * WG + STAGE_Q * WG / 8];
-    __shared__ u32 lhist[FUSE == 1 ? 256 : 1];
-    __shared__ u32 red[16];
-    fuse_init<FUSE>(lhist, bins);
-    u32 mn = 0xFFFFFFFFu, mx = 0u;
-    const u64 G = gridDim.x;
-    u32x4 ra[STAGE_Q], rb[STAGE_Q];
-    auto fetch = [&](u64 t, u32x4 (&r)[STAGE_Q]) {
-        if (t < nfull) {
-            const u32x4* p = (const u32x4*)(keys + t * STAGE_KEYS);
-#pragma unroll
-            for (int u = 0; u < STAGE_Q; ++u) r[u] = load4(p + u * WG + threadIdx.x);
-        }
-    };
-    auto process = [&](u64 t, u32x4 (&r)[STAGE_Q], u64 tn) {
-#pragma unroll
-        for (int u = 0; u < STAGE_Q; ++u) {
-            const int q = u * WG + threadIdx.x;
-            u32x4 v = r[u];
-            v.x = K::to_ord(v.x); v.y = K::to_ord(v.y); v.z = K::to_ord(v.z); v.w = K::to_ord(v.w);
-            tile[q + (q >> 3)] = v;
-        }
-        fetch(tn, r);
-        __syncthreads();
-        staged_tile_full<A, 2, POSM, FUSE, LT>(tile, t * STAGE_KEYS, dvals, dpos, dtop, lhist, hshift, &mn, &mx);
-        __syncthreads();
-    };
-    fetch(blockIdx.x, ra);
-    fetch(blockIdx.x + G, rb);
-    for (u64 t = blockIdx.x; t < nfull; t += 2 * G) {  // uniform per workgroup
-        process(t, ra, t + 2 * G);
-        if (t + G >= nfull) break;
-        process(t + G, rb, t + 3 * G);
-    }
-    fuse_flush<FUSE>(lhist, ghist, bins, mn, mx, ctrl, red);
-}
-
 // ---------------------------------------------------------------- beta 2, lane groups with DPP
 // Beta 2, full 8192-key chunks, alpha 2..5 a compile-time constant, no LDS staging: lane l of a
 // wave holds keys 4l..4l+3 of a 256-key wave tile (eight 16-byte loads in flight per lane), takes
@@ -690,30 +643,10 @@ static hipError_t launch_sample_t(const SampleArgs& a, hipStream_t st) {
     if ((a.staged && a.alpha <= 5) || (a.dtop && !dpp2_ok)) {
         // exactly the resident workgroups (LDS allows 4 per CU): every one keeps its pipeline full
         const u64 ntiles = (a.n + STAGE_KEYS - 1) / STAGE_KEYS;
-        u64 t0 = 0;  // tiles before t0 are done by the beta-2 kernel
-        if constexpr (BETA == 2 && POSM != 1 && FUSE != 0) {
-            const u64 nfull = vec_ok ? a.n / STAGE_KEYS : 0;
-            if (a.alpha >= 2 && nfull > 0) {
-                const unsigned g2 = (unsigned)std::min<u64>(nfull, 4ull * device_cus());
-#define S2_LAUNCH(AA)                                                                                         \
-    hipLaunchKernelGGL((k_sample_stage2<K, POSM, FUSE, AA>), dim3(g2), dim3(WG), 0, st, a.keys, nfull, a.dvals, \
-                       a.dpos, a.ghist, a.hshift, a.bins, a.ctrl, a.dtop)
-                switch (a.alpha) {
-                    case 2: S2_LAUNCH(2); break;
-                    case 3: S2_LAUNCH(3); break;
-                    case 4: S2_LAUNCH(4); break;
-                    default: S2_LAUNCH(5); break;
-                }
-#undef S2_LAUNCH
-                t0 = nfull;
-            }
-        }
-        if (t0 < ntiles) {
-            const unsigned grid = (unsigned)std::min<u64>(ntiles - t0, 4ull * device_cus());
-            hipLaunchKernelGGL((k_sample_staged<K, BETA, POSM, FUSE, LT>), dim3(grid), dim3(WG), 0, st, a.keys, a.n,
-                               a.alpha, vec_ok, t0, ntiles, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl,
-                               BETA == 2 ? a.dtop : nullptr);
-        }
+        const unsigned grid = (unsigned)std::min<u64>(ntiles, 4ull * device_cus());
+        hipLaunchKernelGGL((k_sample_staged<K, BETA, POSM, FUSE, LT>), dim3(grid), dim3(WG), 0, st, a.keys, a.n,
+                           a.alpha, vec_ok, (u64)0, ntiles, a.dvals, a.dpos, a.ghist, a.hshift, a.bins, a.ctrl,
+                           BETA == 2 ? a.dtop : nullptr);
     } else if (dpp2_ok) {
         // lane groups with DPP merges over the full chunks, the staged kernel over the tail
         const u64 nfull = a.n / GROUP_CHUNK;

@@ -80,9 +80,15 @@ typedef enum drtopk_dtype {
 #define DRTOPK_FLAGS_DEFAULT (DRTOPK_FLAG_FILTER | DRTOPK_FLAG_BETA | DRTOPK_FLAG_SHUFFLE)
 
 #define DRTOPK_ALPHA_AUTO (-1)
+/* Radix family: alpha and beta measured on MI355X for this (floor log2 n, ceil log2 k), from the
+ * library's built-in table (drtopksc_amd/autotune.py --single); a cell without an entry, or
+ * another family, uses DRTOPK_ALPHA_AUTO.  No reference counterpart: the reference tuned alpha
+ * by hand per run (alphaManualTuning/main.cu:84). */
+#define DRTOPK_ALPHA_TUNED (-2)
 
 typedef struct drtopk_params {
-    int32_t alpha;        /* log2 of the subrange size; DRTOPK_ALPHA_AUTO = reference policy; 0 = no delegates */
+    int32_t alpha;        /* log2 of the subrange size; DRTOPK_ALPHA_AUTO = reference policy, DRTOPK_ALPHA_TUNED =
+                           * measured table; 0 = no delegates */
     int32_t beta;         /* delegates per subrange on the beta path, 1..4 (reference: argv beta, default 2) */
     int32_t digit_bits;   /* radix digit width, one of 1,2,4,8,16 (reference NBits, default 8) */
     int32_t num_buckets;  /* bucket family: buckets per refinement level, 4..4096 (reference NBucket, 256) */

@@ -79,6 +79,17 @@ def test_full_size_uniform_sweep(orc, dt):
             print(f"[case] full-size uniform sweep k={k} algo={algo}", flush=True)
             gk, gkeys, gidx, st = run(dt, dev, k, algo, ws)
             assert orc.verify_topk(v, k, gk, gkeys, gidx) == 0, (k, algo)
+        # the measured (alpha, beta) table: identical outputs, its own alpha where it has a cell
+        print(f"[case] full-size uniform sweep k={k} tuned", flush=True)
+        pt = dt.params(alpha=dt.ALPHA_TUNED)
+        kth, tk, ti, st = dt.select(dev, k, p=pt, stats=True,
+                                    workspace=torch.empty(dt.workspace_size(n, k, 0, 0, pt), dtype=torch.uint8,
+                                                          device="cuda"))
+        torch.cuda.synchronize()
+        assert orc.verify_topk(v, k, int(kth.cpu().numpy().view(np.uint32)[0]), tk.cpu().numpy().view(np.uint32),
+                               ti.cpu().numpy()) == 0, (k, "tuned")
+        if k in (1 << 16, 1 << 20):
+            assert (st["alpha"], st["beta"]) == ((9, 1) if k == 1 << 16 else (7, 1)), st
     del dev, ws
     torch.cuda.empty_cache()
 
