@@ -369,6 +369,14 @@ def served_leg(dt, torch, nreq: int = 2000, rate: float = 20000.0):
         served_rate = summ(r_rate, t0)
         r_sat, t1 = srv.replay(qs, sat)
         served_sat = summ(r_sat, t1)
+    # launches in flight (worker threads, one stream each): this host gives a process 4 hardware
+    # queues (GPU_MAX_HW_QUEUES), so streams beyond 4 share queues
+    inflight = {}
+    for m in (2, 4, 8, 12):
+        with dt.Server(store, max_inflight=m) as srv:
+            srv.replay(qs[:64], [0] * 64)
+            r_m, tm = srv.replay(qs, sat)
+            inflight[m] = summ(r_m, tm)
     s_rate, t2 = dt.replay_serial(store, qs, arr)
     serial_rate = summ(s_rate, t2)
     s_sat, t3 = dt.replay_serial(store, qs, sat)
@@ -383,6 +391,7 @@ def served_leg(dt, torch, nreq: int = 2000, rate: float = 20000.0):
             "served_at_rate": served_rate, "served_saturated": served_sat,
             "serial_at_rate": serial_rate, "serial_saturated": serial_sat,
             "speedup_vs_serial": round(served_sat["req_s"] / serial_sat["req_s"], 2),
+            "saturated_by_max_inflight": inflight,
             "kth_identical": identical}
 
 

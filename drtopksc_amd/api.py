@@ -64,8 +64,9 @@ class Row(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("alpha", (ctypes.c_uint8 * 32) * 32), ("digit_bits", (ctypes.c_uint8 * 32) * 32),
-                ("tile_log2", (ctypes.c_uint8 * 32) * 32)]
+    """drtopk_tuning: [key type (U32, F32)][ceil log2 n][ceil log2 k] for each field."""
+    _fields_ = [("alpha", ((ctypes.c_uint8 * 32) * 32) * 2), ("digit_bits", ((ctypes.c_uint8 * 32) * 32) * 2),
+                ("tile_log2", ((ctypes.c_uint8 * 32) * 32) * 2)]
 
 
 TUNE_DEFAULT = 0xFF
@@ -278,12 +279,13 @@ def tuning_default() -> Tuning:
 
 
 def tuning_from_table(table: dict) -> Tuning:
-    """Tuning from the autotuner's JSON table: {"cells": [[log2n, log2k, alpha, digit_bits, tile_log2], ...]}."""
+    """Tuning from the autotuner's JSON table:
+    {"cells": [[dtype, log2n, log2k, alpha, digit_bits, tile_log2], ...]}."""
     t = tuning_default()
-    for ln, lk, a, nb, tl in table.get("cells", []):
-        t.alpha[ln][lk] = a
-        t.digit_bits[ln][lk] = nb
-        t.tile_log2[ln][lk] = tl
+    for d, ln, lk, a, nb, tl in table.get("cells", []):
+        t.alpha[d][ln][lk] = a
+        t.digit_bits[d][ln][lk] = nb
+        t.tile_log2[d][ln][lk] = tl
     return t
 
 

@@ -46,7 +46,7 @@ struct BatchHeader {
     u64 magic;
     u64 nrows;
     u64 nchunks;
-    u64 o_rows, o_chunk_row, o_order;
+    u64 o_rows, o_tiles, o_order;
     u64 plan_bytes;
     u64 ws_bytes;
     u64 o_ws_deleg, o_ws_sel;  // workspace byte offsets: delegates, selected subranges
@@ -67,6 +67,15 @@ struct BatchRow {
     u8 alpha, f32, nb, tile_log2;
 };
 static_assert(sizeof(BatchRow) == 64, "row layout");
+
+// One tile of the delegate pass in 16 bytes, so that a workgroup reads a single record per tile
+// (the next tile's while it streams the current one) instead of a tile -> row map entry and then
+// the row's record.
+struct TileDesc {
+    u64 src;  // bits 0..47: first key (index into d_keys); 48..53: alpha; 54: float32 keys
+    u64 dst;  // bits 0..39: first delegate (index into the delegate array); 40..55: length - 1
+};
+static_assert(sizeof(TileDesc) == 16, "tile descriptor layout");
 
 // ---------------------------------------------------------------- delegate pass
 constexpr int BS_NT = 256;
@@ -96,11 +105,19 @@ __device__ __forceinline__ void bsample_tile(const u32* __restrict__ src, u64 le
 #pragma unroll
                 for (int j = 0; j < 4; ++j) q[u][j] = ord<F32>(q[u][j]);
         } else {
+            // the row's last tile: 16-byte loads for every vector inside it, key by key for the
+            // one vector it ends in (the keys after a row may lie outside d_keys)
 #pragma unroll
             for (int u = 0; u < BS_UNR; ++u) {
                 const u64 e = ((u64)(i0 + u) * BS_NT + t) * 4;
+                if (e + 4 <= len) {
+                    q[u] = *(const u32x4*)(src + e);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) q[u][j] = (e + j < len) ? ord<F32>(src[e + j]) : 0u;
+                    for (int j = 0; j < 4; ++j) q[u][j] = ord<F32>(q[u][j]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) q[u][j] = (e + j < len) ? ord<F32>(src[e + j]) : 0u;
+                }
             }
         }
 #pragma unroll
@@ -148,24 +165,28 @@ struct BatchWs {
 __global__ __launch_bounds__(BS_NT) void k_bsample(const unsigned char* __restrict__ plan, const u32* __restrict__ keys,
                                                    BatchWs* wsh, u32* __restrict__ deleg) {
     const BatchHeader* h = (const BatchHeader*)plan;
-    const BatchRow* rows = (const BatchRow*)(plan + h->o_rows);
-    const u32* crow = (const u32*)(plan + h->o_chunk_row);
+    const u32x4* td = (const u32x4*)(plan + h->o_tiles);
     const u64 nch = h->nchunks;
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // the call's first kernel resets the head
         ws_head_begin(&wsh->head);
         wsh->queue = 0;  // row queue of k_brows (next launch)
     }
     __shared__ u32 part[1 << (MAX_TILE_LOG2 - 8)];
+    if (blockIdx.x >= nch) return;
+    u32x4 cur = td[blockIdx.x];
     for (u64 c = blockIdx.x; c < nch; c += gridDim.x) {
-        const BatchRow* rp = rows + crow[c];
-        const int a = rp->alpha, tl = rp->tile_log2;
-        const u64 n = rp->n;
-        const u64 start = (c - rp->chunk_begin) << tl;
-        const u64 len = min((u64)1 << tl, n - start);
-        const u32* src = keys + rp->key_off + start;
-        u32* dout = deleg + rp->deleg_off + (start >> a);
-        if (rp->f32) bsample_tile<true>(src, len, a, dout, part);
+        // the next tile's record, loaded unconditionally (clamped) so nothing waits for it here
+        const u64 cn = c + gridDim.x < nch ? c + gridDim.x : c;
+        const u32x4 nxt = td[cn];
+        const u64 s0 = ((u64)cur.y << 32) | cur.x, d0 = ((u64)cur.w << 32) | cur.z;
+        const int a = (int)((s0 >> 48) & 63);
+        const bool f32 = (s0 >> 54) & 1;
+        const u64 len = ((d0 >> 40) & 0xFFFF) + 1;
+        const u32* src = keys + (s0 & ((1ull << 48) - 1));
+        u32* dout = deleg + (d0 & ((1ull << 40) - 1));
+        if (f32) bsample_tile<true>(src, len, a, dout, part);
         else bsample_tile<false>(src, len, a, dout, part);
+        cur = nxt;
     }
 }
 
@@ -496,18 +517,18 @@ __global__ __launch_bounds__(BR_NT) void k_brows(const unsigned char* __restrict
 void default_tuning(drtopk_tuning* t) {
     std::memset(t, DRTOPK_TUNE_DEFAULT, sizeof(*t));
     for (const auto& c : kTunedCells) {
-        if (c[0] >= 32 || c[1] >= 32) continue;
-        t->alpha[c[0]][c[1]] = c[2];
-        t->digit_bits[c[0]][c[1]] = c[3];
-        t->tile_log2[c[0]][c[1]] = c[4];
+        if (c[0] > 1 || c[1] >= 32 || c[2] >= 32) continue;
+        t->alpha[c[0]][c[1]][c[2]] = c[3];
+        t->digit_bits[c[0]][c[1]][c[2]] = c[4];
+        t->tile_log2[c[0]][c[1]][c[2]] = c[5];
     }
 }
 
 // Default choice for a cell the table leaves open: the reference's alpha policy
 // (baseline+filter+beta+shuffle/main.cu:83-89), 8-bit digits, 8192-key tiles.
-void choose(const drtopk_tuning& tu, u64 n, u64 k, int* alpha, int* nb, int* tile) {
+void choose(const drtopk_tuning& tu, u64 n, u64 k, int f32, int* alpha, int* nb, int* tile) {
     const int ln = ceil_log2(n), lk = ceil_log2(k);
-    const u8 ta = tu.alpha[ln][lk], tn = tu.digit_bits[ln][lk], tt = tu.tile_log2[ln][lk];
+    const u8 ta = tu.alpha[f32][ln][lk], tn = tu.digit_bits[f32][ln][lk], tt = tu.tile_log2[f32][ln][lk];
     int a = ta != DRTOPK_TUNE_DEFAULT ? (int)ta : std::max(0, (floor_log2(n) + 3 - lk) / 2);
     int tl = tt != DRTOPK_TUNE_DEFAULT ? (int)tt : 13;
     tl = std::min(MAX_TILE_LOG2, std::max(10, tl));
@@ -534,19 +555,22 @@ drtopk_status drtopk_batch_plan(const drtopk_row* rows, uint64_t nrows, uint64_t
     clear_error();
     if (!rows || !plan_bytes) return fail(DRTOPK_ERR_INVALID_ARG, "rows and plan_bytes must not be NULL");
     if (nrows == 0 || nrows > (1ull << 24)) return fail(DRTOPK_ERR_INVALID_ARG, "nrows %llu not in [1, 2^24]", (unsigned long long)nrows);
+    if (keys_len >= (1ull << 48)) return fail(DRTOPK_ERR_UNSUPPORTED, "keys_len %llu >= 2^48", (unsigned long long)keys_len);
     drtopk_tuning tu;
     if (tuning) tu = *tuning;
     else default_tuning(&tu);
-    for (int i = 0; i < 32; ++i)
-        for (int j = 0; j < 32; ++j) {
-            const u8 d = tu.digit_bits[i][j], tl = tu.tile_log2[i][j], al = tu.alpha[i][j];
-            if (d != DRTOPK_TUNE_DEFAULT && d != 4 && d != 8)
-                return fail(DRTOPK_ERR_INVALID_ARG, "tuning digit_bits[%d][%d] = %d not 4 or 8", i, j, d);
-            if (tl != DRTOPK_TUNE_DEFAULT && (tl < 10 || tl > MAX_TILE_LOG2))
-                return fail(DRTOPK_ERR_INVALID_ARG, "tuning tile_log2[%d][%d] = %d not in [10, 15]", i, j, tl);
-            if (al != DRTOPK_TUNE_DEFAULT && al > 30)
-                return fail(DRTOPK_ERR_INVALID_ARG, "tuning alpha[%d][%d] = %d not in [0, 30]", i, j, al);
-        }
+    for (int f = 0; f < 2; ++f)
+        for (int i = 0; i < 32; ++i)
+            for (int j = 0; j < 32; ++j) {
+                const u8 d = tu.digit_bits[f][i][j], tl = tu.tile_log2[f][i][j], al = tu.alpha[f][i][j];
+                if (d != DRTOPK_TUNE_DEFAULT && d != 4 && d != 8)
+                    return fail(DRTOPK_ERR_INVALID_ARG, "tuning digit_bits[%d][%d][%d] = %d not 4 or 8", f, i, j, d);
+                if (tl != DRTOPK_TUNE_DEFAULT && (tl < 10 || tl > MAX_TILE_LOG2))
+                    return fail(DRTOPK_ERR_INVALID_ARG, "tuning tile_log2[%d][%d][%d] = %d not in [10, 15]", f, i, j,
+                                tl);
+                if (al != DRTOPK_TUNE_DEFAULT && al > 30)
+                    return fail(DRTOPK_ERR_INVALID_ARG, "tuning alpha[%d][%d][%d] = %d not in [0, 30]", f, i, j, al);
+            }
     std::vector<BatchRow> br(nrows);
     std::vector<u64> cost(nrows);
     u64 nch = 0, dl = 0, sl = 0;
@@ -571,7 +595,7 @@ drtopk_status drtopk_batch_plan(const drtopk_row* rows, uint64_t nrows, uint64_t
             return fail(DRTOPK_ERR_INVALID_ARG, "row %llu: outputs [%llu, +%llu) outside out_len %llu", (unsigned long long)r,
                         (unsigned long long)in.out_offset, (unsigned long long)in.k, (unsigned long long)out_len);
         int a, nb, tl;
-        choose(tu, in.n, in.k, &a, &nb, &tl);
+        choose(tu, in.n, in.k, in.dtype == DRTOPK_F32 ? 1 : 0, &a, &nb, &tl);
         BatchRow& b = br[r];
         std::memset(&b, 0, sizeof(b));
         b.key_off = in.offset;
@@ -598,6 +622,7 @@ drtopk_status drtopk_batch_plan(const drtopk_row* rows, uint64_t nrows, uint64_t
             cost[r] = 4 * in.n;
         }
     }
+    if (dl >= (1ull << 40)) return fail(DRTOPK_ERR_UNSUPPORTED, "batch has 2^40 or more delegates");
     BatchHeader hd;
     std::memset(&hd, 0, sizeof(hd));
     hd.magic = PLAN_MAGIC;
@@ -605,7 +630,7 @@ drtopk_status drtopk_batch_plan(const drtopk_row* rows, uint64_t nrows, uint64_t
     hd.nchunks = nch;
     u64 o = sizeof(BatchHeader);
     hd.o_rows = o; o = align_up(o + sizeof(BatchRow) * nrows, 16);
-    hd.o_chunk_row = o; o = align_up(o + 4 * nch, 16);
+    hd.o_tiles = o; o = align_up(o + sizeof(TileDesc) * nch, 16);
     hd.o_order = o; o = align_up(o + 4 * nrows, 16);
     hd.plan_bytes = o;
     u64 w = 256;
@@ -629,9 +654,17 @@ drtopk_status drtopk_batch_plan(const drtopk_row* rows, uint64_t nrows, uint64_t
     std::memset(P, 0, hd.plan_bytes);
     std::memcpy(P, &hd, sizeof(hd));
     std::memcpy(P + hd.o_rows, br.data(), sizeof(BatchRow) * nrows);
-    u32* crow = (u32*)(P + hd.o_chunk_row);
-    for (u64 r = 0; r < nrows; ++r)
-        for (u32 c = 0; c < br[r].nchunks; ++c) crow[br[r].chunk_begin + c] = (u32)r;
+    TileDesc* tds = (TileDesc*)(P + hd.o_tiles);
+    for (u64 r = 0; r < nrows; ++r) {
+        const BatchRow& b = br[r];
+        for (u32 c = 0; c < b.nchunks; ++c) {
+            const u64 start = (u64)c << b.tile_log2;
+            const u64 len = std::min<u64>(1ull << b.tile_log2, b.n - start);
+            TileDesc& d = tds[b.chunk_begin + c];
+            d.src = (b.key_off + start) | ((u64)b.alpha << 48) | ((u64)b.f32 << 54);
+            d.dst = (b.deleg_off + (start >> b.alpha)) | ((len - 1) << 40);
+        }
+    }
     std::vector<u32> ord(nrows);
     std::iota(ord.begin(), ord.end(), 0u);
     std::stable_sort(ord.begin(), ord.end(), [&](u32 x, u32 y) { return cost[x] > cost[y]; });

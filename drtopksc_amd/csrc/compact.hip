@@ -637,7 +637,6 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
     h2_init<HIST2>(lh, bins2);
     const u32 n = (u32)n64;
     const u32 thr = ctrl->first.result;
-    H2Direct h2d{thr >> hshift2};
     const u32 L = 1u << alpha;
     const u32 S = (u32)((n64 + L - 1) >> alpha);
     const u32 cps = (u32)cps64;
@@ -879,7 +878,7 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
                 } else {
                     gk[slot] = o;
                     if (POS) gi[slot] = gidx;
-                    if (HIST2) h2d.add(lh, o >> hshift2);
+                    if (HIST2) atomicAdd(&lh[o >> hshift2], 1u);
                 }
             };
             if (BETA == 2 && (filter & 2)) {
@@ -990,7 +989,6 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
             if (dd) atomicAdd((unsigned long long*)&ctrl->lonely_dropped, (unsigned long long)dd);
         }
     }
-    if (HIST2) h2d.flush(lh);
     h2_flush<HIST2>(lh, ghist2, bins2);
 }
 

@@ -213,15 +213,17 @@ typedef struct drtopk_row {
 } drtopk_row;
 
 #define DRTOPK_TUNE_DEFAULT 0xFFu
-/* Per-(ceil log2 n, ceil log2 k) bucket choices, as written by the project's autotuner
- * (drtopksc_amd/autotune.py).  DRTOPK_TUNE_DEFAULT in a cell keeps the library's built-in
- * choice.  alpha 0 = no delegates (the row is selected directly); the planner lowers an alpha
- * that leaves fewer than k subranges or exceeds the tile.  digit_bits 4 or 8; tile_log2 10..15
- * is log2 of the keys one workgroup of the delegate pass streams per work unit. */
+/* Per-(key type, ceil log2 n, ceil log2 k) bucket choices, as written by the project's autotuner
+ * (drtopksc_amd/autotune.py); the first index is the row's drtopk_dtype (0 uint32, 1 float32),
+ * whose keys are distributed differently (float32: sign and exponent in the top digits).
+ * DRTOPK_TUNE_DEFAULT in a cell keeps the library's built-in choice.  alpha 0 = no delegates (the
+ * row is selected directly); the planner lowers an alpha that leaves fewer than k subranges or
+ * exceeds the tile.  digit_bits 4 or 8; tile_log2 10..15 is log2 of the keys one workgroup of
+ * the delegate pass streams per work unit. */
 typedef struct drtopk_tuning {
-    uint8_t alpha[32][32];
-    uint8_t digit_bits[32][32];
-    uint8_t tile_log2[32][32];
+    uint8_t alpha[2][32][32];
+    uint8_t digit_bits[2][32][32];
+    uint8_t tile_log2[2][32][32];
 } drtopk_tuning;
 
 /* Plan a batch.  Host only.  Call with h_plan == NULL to get *plan_bytes; then again with a host

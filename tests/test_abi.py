@@ -159,15 +159,16 @@ def test_batch_plan_validation(L):
     assert plan([(0, 5000, 10, 0, 0)], keys_len=4000) == 1   # row outside the key buffer
     assert plan([(0, 5000, 10, 0, 0)], out_len=5) == 1       # outputs outside the output arrays
     t = api.tuning_default()
-    t.digit_bits[13][4] = 5
+    t.digit_bits[0][13][4] = 5
     assert plan([(0, 5000, 10, 0, 0)], tuning=t) == 1 and "digit_bits" in L.drtopk_last_error().decode()
     t = api.tuning_default()
-    t.tile_log2[13][4] = 9
+    t.tile_log2[1][13][4] = 9
     assert plan([(0, 5000, 10, 0, 0)], tuning=t) == 1
 
 
 def test_batch_plan_layout(L):
-    """The plan blob: header, one 64-byte record per row, the tile -> row map, the row order."""
+    """The plan blob: header, one 64-byte record per row, one record per delegate-pass tile, the row
+    order."""
     import struct
 
     import numpy as np
@@ -194,9 +195,17 @@ def test_batch_plan_layout(L):
             tiles += nc
         assert nb in (4, 8)
     assert nchunks == tiles
-    crow = np.frombuffer(buf.tobytes(), np.uint32, nchunks, o_crow)
-    for r, rec in enumerate(recs):
-        assert np.all(crow[rec[7]:rec[7] + rec[8]] == r)
+    # one 16-byte record per delegate-pass tile: first key | alpha << 48 | f32 << 54, then first
+    # delegate | (length - 1) << 40
+    td = np.frombuffer(buf.tobytes(), np.uint64, 2 * nchunks, o_crow).reshape(nchunks, 2)
+    for rec in recs:
+        key_off, n, alpha, f32, tl, deleg_off, cb, nc = rec[0], rec[1], rec[9], rec[10], rec[12], rec[4], rec[7], rec[8]
+        for c in range(nc):
+            start = c << tl
+            src, dst = int(td[cb + c, 0]), int(td[cb + c, 1])
+            assert src & ((1 << 48) - 1) == key_off + start and (src >> 48) & 63 == alpha and (src >> 54) & 1 == f32
+            assert dst & ((1 << 40) - 1) == deleg_off + (start >> alpha)
+            assert (dst >> 40) + 1 == min(1 << tl, n - start)
     order = np.frombuffer(buf.tobytes(), np.uint32, 3, o_order)
     assert sorted(order.tolist()) == [0, 1, 2] and order[0] == 2  # largest row first
 

@@ -101,11 +101,12 @@ def test_batch_tuning_overrides(orc, dt, alpha, nb, tile):
                                                    (1 << 18, 1, 0, "rand"), (33333, 2048, 1, "equal")]]
     rows, buf, kl, ol, ordered = make_rows(orc, specs, seed0=2000 + alpha)
     t = dt.tuning_default()
-    for ln in range(32):
-        for lk in range(32):
-            t.alpha[ln][lk] = alpha
-            t.digit_bits[ln][lk] = nb
-            t.tile_log2[ln][lk] = tile
+    for d in range(2):
+        for ln in range(32):
+            for lk in range(32):
+                t.alpha[d][ln][lk] = alpha
+                t.digit_bits[d][ln][lk] = nb
+                t.tile_log2[d][ln][lk] = tile
     print(f"[case] batch tuning alpha={alpha} nb={nb} tile={tile}", flush=True)
     _, _, kth, tk, ti = run_batch(dt, rows, buf, kl, ol, tuning=t)
     check(orc, rows, ordered, kth, tk, ti, f"tune a{alpha} nb{nb} t{tile}")

@@ -18,6 +18,18 @@ torch = pytest.importorskip("torch")
 KS = (1, 1 << 10, 1 << 16, 1 << 20, 1 << 21, 1 << 22, 1 << 24)
 
 
+def _tuned_2p30():
+    """(alpha, beta) the measured table (drtopksc_amd/tuning/mi355x_single.json) picks at n = 2^30."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "drtopksc_amd", "tuning",
+                        "mi355x_single.json")
+    return {1 << lk: (a, b) for ln, lk, a, b in json.load(open(path))["single"] if ln == 30}
+
+
+TUNED_2P30 = _tuned_2p30()
+
+
 @pytest.fixture(scope="module")
 def dt():
     if not torch.cuda.is_available():
@@ -88,8 +100,7 @@ def test_full_size_uniform_sweep(orc, dt):
         torch.cuda.synchronize()
         assert orc.verify_topk(v, k, int(kth.cpu().numpy().view(np.uint32)[0]), tk.cpu().numpy().view(np.uint32),
                                ti.cpu().numpy()) == 0, (k, "tuned")
-        if k in (1 << 16, 1 << 20):
-            assert (st["alpha"], st["beta"]) == ((9, 1) if k == 1 << 16 else (7, 1)), st
+        assert (st["alpha"], st["beta"]) == TUNED_2P30[k], (k, st["alpha"], st["beta"])
     del dev, ws
     torch.cuda.empty_cache()
 
