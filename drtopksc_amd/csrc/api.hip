@@ -69,6 +69,7 @@ struct Plan {
     // byte offsets
     u64 o_ctrl, o_hist, o_zero_end;
     u64 o_rcnt, o_roff, o_ecnt, o_eoff, o_scnt;
+    u64 npart, o_part;  // beta path: per-workgroup records of the concatenation
     u64 o_dvals, o_doff, o_sel, o_skeys, o_sidx, total;
     u64 npad1, npad2, o_bufA, o_bufB, o_dsel;  // bitonic family
 };
@@ -184,6 +185,8 @@ drtopk_status resolve(drtopk_algo algo, drtopk_dtype dtype, u64 n, u64 k, const 
     pl->o_ecnt = o; o = align_up(o + 8ull * pl->ext_nseg, 256);
     pl->o_eoff = o; o = align_up(o + 8ull * pl->ext_nseg, 256);
     pl->o_scnt = o; o = align_up(o + 4ull * pl->nseg, 256);
+    pl->npart = (path == 2 && !bit) ? beta_grid(pl->S, alpha) : 0;
+    pl->o_part = o; o = align_up(o + 32ull * pl->npart, 256);
     pl->o_dvals = o; o = align_up(o + 4ull * pl->D, 256);
     const u64 offb = path == 2 ? (beta == 2 ? 1 : beta == 3 ? 2 : 4) : 0;
     pl->o_doff = o; o = align_up(o + offb * pl->S, 256);
@@ -272,6 +275,7 @@ drtopk_status select_impl(const Plan& pl, const void* d_keys, void* d_workspace,
     u32* sel_ids = (u32*)(ws + pl.o_sel);
     u32* skeys = (u32*)(ws + pl.o_skeys);
     u32* sidx = (u32*)(ws + pl.o_sidx);
+    u64* part = pl.npart ? (u64*)(ws + pl.o_part) : nullptr;
     const u32* keys = (const u32*)d_keys;
     const bool want_out = d_topk_keys || d_topk_idx;
     const bool radix = pl.algo == DRTOPK_ALGO_RADIX;
@@ -417,13 +421,13 @@ drtopk_status select_impl(const Plan& pl, const void* d_keys, void* d_workspace,
             // threshold are counted, not written
             const int bfilter = (pl.flags & DRTOPK_FLAG_FILTER) ? (radix ? 3 : 1) : 0;
             BetaGatherArgs ba{keys, pl.n, pl.alpha, pl.beta, pl.f32, bfilter,
-                              dvals, doff, ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins, dtop, seg_limit};
+                              dvals, doff, ctrl, skeys, sidx_or_null, scnt, h2, 32 - nb, bins, dtop, seg_limit, part};
             HIPCHK(launch_beta_gather(ba, st));
         }
         // with the first digit's histogram fused into the concatenation, the first select also
         // totals the segments; otherwise a one-workgroup pass does before the first histogram
         const bool total_in_select = radix && fuse_hist;
-        if (!total_in_select) HIPCHK(launch_seg_total(scnt, pl.nseg, ctrl, st));
+        if (!total_in_select) HIPCHK(launch_seg_total(scnt, pl.nseg, ctrl, st, part, pl.npart));
         HIPCHK(mark(3));
         // ---- second top-k over the concatenation (radix_select_inplace :1891 / bucket_select :1355)
         const Seg cseg{skeys, sidx, scnt, pl.segcap, pl.nseg, 0};
@@ -431,7 +435,8 @@ drtopk_status select_impl(const Plan& pl, const void* d_keys, void* d_workspace,
         if (radix) {
             for (int d = 0; d <= D0; ++d) {
                 if (!(d == 0 && fuse_hist)) HIPCHK(launch_hist_seg(0, cseg, sec, hist, bins, gc, st));
-                if (d == 0 && total_in_select) HIPCHK(launch_select(0, hist, bins, sec, ctrl, st, scnt, pl.nseg));
+                if (d == 0 && total_in_select)
+                    HIPCHK(launch_select(0, hist, bins, sec, ctrl, st, scnt, pl.nseg, part, pl.npart));
                 else HIPCHK(launch_select(0, hist, bins, sec, ctrl, st));
             }
         } else {

@@ -229,6 +229,46 @@ __device__ __forceinline__ void wave_hist_add(u32* lhist, bool active, u32 b) {
     if (rem) atomicAdd(&lhist[b], 1u);
 }
 
+// ---------------------------------------------------------------- beta concatenation counts
+// The beta-path concatenation leaves one record per workgroup: (selected subranges, lonely
+// subranges, lonely delegates appended, lonely delegates dropped).  Thousands of workgroups
+// adding them to four Ctrl words would queue on one L2 line at the kernel's end (40 us at 2048
+// workgroups measured); the pass that totals the segment counts sums the records instead.
+// Every thread of an NT-thread block calls this; thread 0 writes the Ctrl fields.
+template <int NT>
+__device__ __forceinline__ void beta_part_total(const u64* __restrict__ part, u64 npart, Ctrl* ctrl,
+                                                u64 (*lds)[NT / WAVE]) {
+    // one workgroup reads every record: unrolled so that a thread's loads are all in flight
+    // together (a loop that waits for each record costs a memory latency per record)
+    u64 s[4] = {0, 0, 0, 0};
+    const ulonglong2* p2 = (const ulonglong2*)part;  // record i: p2[2i], p2[2i + 1] (32-byte aligned)
+#pragma unroll 8
+    for (u64 i = threadIdx.x; i < npart; i += NT) {
+        const ulonglong2 a = p2[2 * i], b = p2[2 * i + 1];
+        s[0] += a.x;
+        s[1] += a.y;
+        s[2] += b.x;
+        s[3] += b.y;
+    }
+    const int w = threadIdx.x / WAVE;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        s[q] = wave_sum_u64(s[q]);
+        if ((threadIdx.x & (WAVE - 1)) == 0) lds[q][w] = s[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 t[4] = {0, 0, 0, 0};
+        for (int i = 0; i < NT / WAVE; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) t[q] += lds[q][i];
+        ctrl->selected = t[0];
+        ctrl->lonely_sub = t[1];
+        ctrl->lonely_el = t[2];
+        ctrl->lonely_dropped = t[3];
+    }
+}
+
 // ---------------------------------------------------------------- packed counts
 // Pack two 31-bit counts for the look-back payload.
 __device__ __forceinline__ u64 pack2(u64 hi, u64 lo) { return (hi << 31) | lo; }

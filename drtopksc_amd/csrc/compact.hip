@@ -625,7 +625,8 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
                                                     const u32* __restrict__ dvals, const void* __restrict__ doff,
                                                     u64 cps64, Ctrl* ctrl, u32* __restrict__ skeys,
                                                     u32* __restrict__ sidx, u32* __restrict__ cnt, u32* ghist2,
-                                                    int hshift2, int bins2, const u8* __restrict__ dtop, u32 cap) {
+                                                    int hshift2, int bins2, const u8* __restrict__ dtop, u32 cap,
+                                                    u64* __restrict__ part) {
     __shared__ u32 scratch[WG / WAVE];
     __shared__ u32 lh[HIST2 ? 256 : 1];
     __shared__ u64 red[4][WG / WAVE];
@@ -980,13 +981,14 @@ __global__ __launch_bounds__(WG, WAVES) void k_beta_gather(const u32* __restrict
             if (lane == 0) red[q][w] = a4[q];
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0) {  // this workgroup's record (summed by beta_part_total)
             u64 a = 0, b = 0, cc = 0, dd = 0;
             for (int i = 0; i < WG / WAVE; ++i) { a += red[0][i]; b += red[1][i]; cc += red[2][i]; dd += red[3][i]; }
-            if (a) atomicAdd((unsigned long long*)&ctrl->selected, (unsigned long long)a);
-            if (b) atomicAdd((unsigned long long*)&ctrl->lonely_sub, (unsigned long long)b);
-            if (cc) atomicAdd((unsigned long long*)&ctrl->lonely_el, (unsigned long long)cc);
-            if (dd) atomicAdd((unsigned long long*)&ctrl->lonely_dropped, (unsigned long long)dd);
+            u64* r = part + 4ull * blockIdx.x;
+            r[0] = a;
+            r[1] = b;
+            r[2] = cc;
+            r[3] = dd;
         }
     }
     h2_flush<HIST2>(lh, ghist2, bins2);
@@ -1033,6 +1035,7 @@ u64 beta_nseg(u64 S, int alpha) {
     const u64 cps = beta_cps(nch, alpha);
     return (nch + cps - 1) / cps;
 }
+u64 beta_grid(u64 S, int alpha) { return std::min<u64>(beta_nseg(S, alpha), CGRID); }
 u64 beta_segcap(u64 S, int alpha) {
     return beta_cps((S + BETA_TILE_SUBS - 1) / BETA_TILE_SUBS, alpha) * (BETA_TILE_SUBS << alpha);
 }
@@ -1045,10 +1048,11 @@ static hipError_t launch_beta_t(const BetaGatherArgs& a, hipStream_t st) {
     const u64 cps = beta_cps(nch, a.alpha);
     const u64 nseg = (nch + cps - 1) / cps;
     const unsigned grid = (unsigned)std::min<u64>(nseg, CGRID);
+    if (grid != beta_grid(S, a.alpha)) return hipErrorInvalidValue;  // a.part holds beta_grid records
     const u64 segcap = cps * (BETA_TILE_SUBS << a.alpha);
     const u32 cap = (u32)std::min<u64>(segcap, a.seg_limit ? a.seg_limit : segcap);
 #define B_ARGS a.keys, a.n, a.alpha, a.filter, a.dvals, a.doff, cps, a.ctrl, a.skeys, a.sidx, a.cnt, a.ghist2, \
-               a.hshift2, a.bins2, BETA == 2 ? a.dtop : nullptr, cap
+               a.hshift2, a.bins2, BETA == 2 ? a.dtop : nullptr, cap, a.part
     const bool pos = a.sidx != nullptr, h = a.ghist2 != nullptr;
 #define B_LAUNCH(W, KB)                                                                                          \
     do {                                                                                                         \
@@ -1080,9 +1084,13 @@ hipError_t launch_beta_gather(const BetaGatherArgs& a, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------- concatenation size
-__global__ __launch_bounds__(SCAN_T) void k_seg_total(const u32* __restrict__ cnt, u64 nseg, Ctrl* ctrl) {
+__global__ __launch_bounds__(SCAN_T) void k_seg_total(const u32* __restrict__ cnt, u64 nseg, Ctrl* ctrl,
+                                                     const u64* __restrict__ part, u64 npart) {
     __shared__ u64 scratch[SCAN_T / WAVE + 1];
+    __shared__ u64 pl[4][SCAN_T / WAVE];
+    if (part) beta_part_total<SCAN_T>(part, npart, ctrl, pl);
     u64 s = 0;
+#pragma unroll 8
     for (u64 i = threadIdx.x; i < nseg; i += SCAN_T) s += cnt[i];
     u64 tot;
     block_excl_scan_u64<SCAN_T>(s, scratch, &tot);
@@ -1095,8 +1103,8 @@ __global__ __launch_bounds__(SCAN_T) void k_seg_total(const u32* __restrict__ cn
         }
     }
 }
-hipError_t launch_seg_total(const u32* cnt, u64 nseg, Ctrl* ctrl, hipStream_t st) {
-    hipLaunchKernelGGL(k_seg_total, dim3(1), dim3(SCAN_T), 0, st, cnt, nseg, ctrl);
+hipError_t launch_seg_total(const u32* cnt, u64 nseg, Ctrl* ctrl, hipStream_t st, const u64* part, u64 npart) {
+    hipLaunchKernelGGL(k_seg_total, dim3(1), dim3(SCAN_T), 0, st, cnt, nseg, ctrl, part, npart);
     return hipGetLastError();
 }
 

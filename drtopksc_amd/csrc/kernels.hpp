@@ -49,9 +49,10 @@ hipError_t launch_hist_raw(int mode, int f32, const u32* keys, SelState* sel, u3
 hipError_t launch_hist_seg(int mode, const Seg& seg, SelState* sel, u32* ghist, int bins, unsigned grid,
                            hipStream_t st);
 // segcnt / nseg: first pass over a segmented concatenation; the kernel then also sets its size
-// (concat_size, sel->m and the too-short error) from the segment counts.
+// (concat_size, sel->m and the too-short error) from the segment counts, and with `part` first
+// sums the beta concatenation's npart per-workgroup records (beta_part_total).
 hipError_t launch_select(int mode, u32* ghist, int bins, SelState* sel, Ctrl* ctrl, hipStream_t st,
-                         const u32* segcnt = nullptr, u64 nseg = 0);
+                         const u32* segcnt = nullptr, u64 nseg = 0, const u64* part = nullptr, u64 npart = 0);
 hipError_t launch_bucket_begin(SelState* sel, Ctrl* ctrl, int from_ctrl_minmax, hipStream_t st);
 hipError_t launch_minmax(const u32* arr, SelState* sel, Ctrl* ctrl, unsigned grid, hipStream_t st);
 hipError_t launch_minmax_raw(int f32, const u32* keys, SelState* sel, Ctrl* ctrl, unsigned grid,
@@ -114,15 +115,19 @@ struct BetaGatherArgs {
     int hshift2, bins2;
     const u8* dtop = nullptr;  // beta 2: delegates in the sampler's dtop form (dvals[s] first, 16-bit dtop words)
     u64 seg_limit = 0;         // entries a segment may hold (0: its capacity)
+    u64* part = nullptr;       // [beta_grid] per-workgroup records (selected, lonely, lonely delegates, dropped)
 };
 constexpr int BETA_SUBS_PER_THREAD = 4;  // 4 measured faster than 8 (fewer VGPRs, more resident workgroups)
 constexpr u64 BETA_TILE_SUBS = 256 * BETA_SUBS_PER_THREAD;  // subranges per chunk of the beta gather
 hipError_t launch_beta_gather(const BetaGatherArgs& a, hipStream_t st);
 u64 beta_segcap(u64 S, int alpha);
 u64 beta_nseg(u64 S, int alpha);
+u64 beta_grid(u64 S, int alpha);  // workgroups of the beta gather = records it writes
 
-// concatenation size = sum of segment counts -> ctrl->concat_size, second.m
-hipError_t launch_seg_total(const u32* cnt, u64 nseg, Ctrl* ctrl, hipStream_t st);
+// concatenation size = sum of segment counts -> ctrl->concat_size, second.m (with `part`: the
+// beta concatenation's per-workgroup records summed into the Ctrl counts first)
+hipError_t launch_seg_total(const u32* cnt, u64 nseg, Ctrl* ctrl, hipStream_t st, const u64* part = nullptr,
+                            u64 npart = 0);
 
 // findTopk + FillTopk: keys above the k-th largest, then the lowest-index ties, index order.
 struct ExtractArgs {

@@ -196,14 +196,18 @@ constexpr int SEL_T = 256;
 
 template <int MODE>
 __global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState* sel, Ctrl* ctrl,
-                                                 const u32* __restrict__ segcnt, u64 nseg) {
+                                                 const u32* __restrict__ segcnt, u64 nseg,
+                                                 const u64* __restrict__ part, u64 npart) {
     __shared__ u64 scratch[SEL_T / WAVE + 1];
     __shared__ u64 found[3];  // bin, count before it, count in it
     u64 seg_m = 0;  // with segcnt: the concatenation's size (every thread holds it)
     if (segcnt) {
+        __shared__ u64 pl[4][SEL_T / WAVE];
+        if (part) beta_part_total<SEL_T>(part, npart, ctrl, pl);
         // first pass over a segmented concatenation: its size from the segment counts first
         // (what a separate one-workgroup total would do, see k_seg_total)
         u64 s = 0;
+#pragma unroll 8
         for (u64 i = threadIdx.x; i < nseg; i += SEL_T) s += segcnt[i];
         u64 tot;
         block_excl_scan_u64<SEL_T>(s, scratch, &tot);
@@ -330,9 +334,11 @@ __global__ __launch_bounds__(SEL_T) void k_select(u32* ghist, int bins, SelState
 }
 
 hipError_t launch_select(int mode, u32* ghist, int bins, SelState* sel, Ctrl* ctrl, hipStream_t st,
-                         const u32* segcnt, u64 nseg) {
-    if (mode == 0) hipLaunchKernelGGL(k_select<0>, dim3(1), dim3(SEL_T), 0, st, ghist, bins, sel, ctrl, segcnt, nseg);
-    else hipLaunchKernelGGL(k_select<1>, dim3(1), dim3(SEL_T), 0, st, ghist, bins, sel, ctrl, segcnt, nseg);
+                         const u32* segcnt, u64 nseg, const u64* part, u64 npart) {
+    if (mode == 0)
+        hipLaunchKernelGGL(k_select<0>, dim3(1), dim3(SEL_T), 0, st, ghist, bins, sel, ctrl, segcnt, nseg, part, npart);
+    else
+        hipLaunchKernelGGL(k_select<1>, dim3(1), dim3(SEL_T), 0, st, ghist, bins, sel, ctrl, segcnt, nseg, part, npart);
     return hipGetLastError();
 }
 
