@@ -460,8 +460,10 @@ __device__ __forceinline__ void staged_tile_full(const u32x4* tile, u64 k0, u32*
     }
 }
 
+// At most 128 VGPRs (4 waves per SIMD): the LDS tile allows 4 workgroups per CU and the grid is
+// sized for exactly that many (at 130 VGPRs one workgroup in four waited for a second round).
 template <class K, int BETA, int POSM, int FUSE, class LT>
-__global__ __launch_bounds__(WG) void k_sample_staged(const u32* __restrict__ keys, u64 n, int alpha, int vec_ok,
+__global__ __launch_bounds__(WG, 4) void k_sample_staged(const u32* __restrict__ keys, u64 n, int alpha, int vec_ok,
                                                       u64 t_begin, u64 ntiles, u32* __restrict__ dvals,
                                                       void* __restrict__ dpos, u32* __restrict__ ghist, int hshift,
                                                       int bins, Ctrl* ctrl, u8* __restrict__ dtop) {
