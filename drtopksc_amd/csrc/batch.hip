@@ -87,29 +87,53 @@ __device__ __forceinline__ u32 ord(u32 b) {
     return F32 ? KeyF32::to_ord(b) : b;
 }
 
-// One tile (<= 2^15 keys) of one row: subrange maxima into dout.
+// Maximum over each aligned group of g lanes (g = 1, 2, 4, .., 64, uniform over the wave); every
+// lane of a group ends with its group's maximum.  DPP moves inside rows of 16 lanes (quad
+// permutes, then the half-row and row mirrors), LDS permutes across rows.
+__device__ __forceinline__ u32 group_max(u32 m, int g) {
+    if (g >= 2) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    if (g >= 4) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    if (g >= 8) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    if (g >= 16) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x140, 0xF, 0xF, false)); // row_mirror
+    if (g >= 32) m = max(m, (u32)__shfl_xor((int)m, 16, WAVE));
+    if (g >= 64) m = max(m, (u32)__shfl_xor((int)m, 32, WAVE));
+    return m;
+}
+
+__device__ __forceinline__ u32 max4(const u32x4& v) { return max(max(v[0], v[1]), max(v[2], v[3])); }
+
+constexpr u32 BS_WKEYS = BS_UNR * WAVE * 4;  // keys per wave per batch (2048)
+constexpr u32 BS_BATCH = BS_WKEYS * (BS_NT / WAVE);  // keys per workgroup per batch (8192)
+
+// One tile (<= 2^15 keys) of one row: subrange maxima into dout.  Each wave owns 2048
+// consecutive keys per batch (eight 16-byte loads per lane, 1 KB per load instruction), so a
+// subrange of up to 2^11 keys lies inside one wave: alpha 2..8 reduce within lane groups,
+// alpha 9..11 across 2^(alpha-8) of the wave's loads, with no barrier.  Longer subranges
+// (alpha 12..15) combine the waves' 2048-key maxima through LDS.
 template <bool F32>
 __device__ __forceinline__ void bsample_tile(const u32* __restrict__ src, u64 len, int a, u32* __restrict__ dout,
                                              u32* part) {
-    const int t = threadIdx.x, lane = t & (WAVE - 1);
-    const u32 nvec = (u32)((len + 3) >> 2);
-    const u32 iters = (nvec + BS_NT - 1) / BS_NT;
-    for (u32 i0 = 0; i0 < iters; i0 += BS_UNR) {
+    const int t = threadIdx.x, lane = t & (WAVE - 1), w = t / WAVE;
+    const u32 nbat = (u32)((len + BS_BATCH - 1) / BS_BATCH);
+    for (u32 b = 0; b < nbat; ++b) {
+        const u64 wb = (u64)b * BS_BATCH + (u64)w * BS_WKEYS;  // the wave's first key
+        if (wb >= len) break;  // uniform over the wave; no barrier below
         u32x4 q[BS_UNR];
-        if ((u64)(i0 + BS_UNR) * BS_NT * 4 <= len) {
+        if (wb + BS_WKEYS <= len) {
 #pragma unroll
             for (int u = 0; u < BS_UNR; ++u)
-                q[u] = __builtin_nontemporal_load((const u32x4*)(src + ((u64)(i0 + u) * BS_NT + t) * 4));
+                q[u] = __builtin_nontemporal_load((const u32x4*)(src + wb + (u64)(u * WAVE + lane) * 4));
 #pragma unroll
             for (int u = 0; u < BS_UNR; ++u)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) q[u][j] = ord<F32>(q[u][j]);
         } else {
             // the row's last tile: 16-byte loads for every vector inside it, key by key for the
-            // one vector it ends in (the keys after a row may lie outside d_keys)
+            // one vector it ends in (the keys after a row may lie outside d_keys); 0 is below
+            // every ordered key
 #pragma unroll
             for (int u = 0; u < BS_UNR; ++u) {
-                const u64 e = ((u64)(i0 + u) * BS_NT + t) * 4;
+                const u64 e = wb + (u64)(u * WAVE + lane) * 4;
                 if (e + 4 <= len) {
                     q[u] = *(const u32x4*)(src + e);
 #pragma unroll
@@ -120,36 +144,54 @@ __device__ __forceinline__ void bsample_tile(const u32* __restrict__ src, u64 le
                 }
             }
         }
+        if (a == 1) {
 #pragma unroll
-        for (int u = 0; u < BS_UNR; ++u) {
-            if (i0 + u >= iters) break;  // uniform over the workgroup
-            const u64 e = ((u64)(i0 + u) * BS_NT + t) * 4;
-            if (a == 1) {
+            for (int u = 0; u < BS_UNR; ++u) {
+                const u64 e = wb + (u64)(u * WAVE + lane) * 4;
                 if (e < len) dout[e >> 1] = max(q[u][0], q[u][1]);
                 if (e + 2 < len) dout[(e >> 1) + 1] = max(q[u][2], q[u][3]);
-            } else if (a <= 8) {
-                u32 m = max(max(q[u][0], q[u][1]), max(q[u][2], q[u][3]));
-                const int g = 1 << (a - 2);  // lanes per subrange
-                for (int o = 1; o < g; o <<= 1) m = max(m, (u32)__shfl_xor((int)m, o, WAVE));
-                if ((lane & (g - 1)) == 0 && e < len) dout[e >> a] = m;
-            } else {
-                const u32 m = wave_max_u32(max(max(q[u][0], q[u][1]), max(q[u][2], q[u][3])));
-                if (lane == 0) part[((i0 + u) * BS_NT + t) >> 6] = m;  // 256-key block maximum
             }
+        } else if (a <= 8) {
+            const int g = 1 << (a - 2);  // lanes per subrange
+#pragma unroll
+            for (int u = 0; u < BS_UNR; ++u) {
+                const u64 e = wb + (u64)(u * WAVE + lane) * 4;
+                const u32 m = group_max(max4(q[u]), g);
+                if ((lane & (g - 1)) == 0 && e < len) dout[e >> a] = m;
+            }
+        } else if (a <= 11) {
+            const int per = 1 << (a - 8);  // loads per subrange
+            u32 acc = 0;
+#pragma unroll
+            for (int u = 0; u < BS_UNR; ++u) {
+                acc = max(acc, max4(q[u]));
+                if (((u + 1) & (per - 1)) == 0) {
+                    const u64 e0 = wb + (u64)(u + 1 - per) * WAVE * 4;  // the subrange's first key
+                    const u32 m = group_max(acc, WAVE);
+                    if (lane == 0 && e0 < len) dout[e0 >> a] = m;
+                    acc = 0;
+                }
+            }
+        } else {
+            u32 acc = 0;
+#pragma unroll
+            for (int u = 0; u < BS_UNR; ++u) acc = max(acc, max4(q[u]));
+            acc = group_max(acc, WAVE);
+            if (lane == 0) part[wb / BS_WKEYS] = acc;  // the wave's 2048-key maximum
         }
     }
-    if (a > 8) {
+    if (a >= 12) {
         __syncthreads();
-        const u32 per = 1u << (a - 8);
-        const u32 nblk = (u32)((len + 255) >> 8);
+        const u32 per = 1u << (a - 11);
+        const u32 nblk = (u32)((len + BS_WKEYS - 1) / BS_WKEYS);
         const u32 nsub = (u32)((len + (1ull << a) - 1) >> a);
-        for (u32 s = t; s < nsub; s += BS_NT) {
+        for (u32 sb = t; sb < nsub; sb += BS_NT) {
             u32 m = 0;
             for (u32 j = 0; j < per; ++j) {
-                const u32 b = s * per + j;
-                if (b < nblk) m = max(m, part[b]);
+                const u32 bk = sb * per + j;
+                if (bk < nblk) m = max(m, part[bk]);
             }
-            dout[s] = m;
+            dout[sb] = m;
         }
         __syncthreads();
     }
@@ -171,7 +213,7 @@ __global__ __launch_bounds__(BS_NT) void k_bsample(const unsigned char* __restri
         ws_head_begin(&wsh->head);
         wsh->queue = 0;  // row queue of k_brows (next launch)
     }
-    __shared__ u32 part[1 << (MAX_TILE_LOG2 - 8)];
+    __shared__ u32 part[1 << (MAX_TILE_LOG2 - 11)];  // 2048-key block maxima (alpha >= 12)
     if (blockIdx.x >= nch) return;
     u32x4 cur = td[blockIdx.x];
     for (u64 c = blockIdx.x; c < nch; c += gridDim.x) {
@@ -639,6 +681,7 @@ drtopk_status drtopk_batch_plan(const drtopk_row* rows, uint64_t nrows, uint64_t
     hd.ws_bytes = w;
     hd.keys_len = keys_len;
     hd.out_len = out_len;
+    // 4096 workgroups (2.7 rounds of the 1536 resident ones): 1% faster than exactly one round
     hd.sample_grid = (u32)std::max<u64>(1, std::min<u64>(nch, 4096));
     hd.rows_grid = (u32)std::min<u64>(nrows, 512);
     if (ws_bytes) *ws_bytes = (size_t)hd.ws_bytes;

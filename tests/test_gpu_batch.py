@@ -92,8 +92,9 @@ def test_batch_edges(orc, dt):
     check(orc, rows, ordered, kth, tk, ti, "edges")
 
 
-@pytest.mark.parametrize("alpha,nb,tile", [(0, 8, 13), (1, 8, 13), (2, 4, 10), (5, 8, 15), (9, 4, 13),
-                                           (12, 8, 12), (15, 8, 15)])
+@pytest.mark.parametrize("alpha,nb,tile", [(0, 8, 13), (1, 8, 13), (2, 4, 10), (3, 8, 11), (4, 8, 13), (5, 8, 15),
+                                           (6, 8, 14), (7, 4, 13), (8, 8, 15), (9, 4, 13), (10, 8, 13), (11, 8, 15),
+                                           (12, 8, 12), (13, 8, 13), (14, 8, 15), (15, 8, 15)])
 def test_batch_tuning_overrides(orc, dt, alpha, nb, tile):
     """Every alpha / digit width / tile the planner accepts gives the same rows."""
     specs = [(n, k, d, kind) for n, k, d, kind in [(1 << 16, 100, 0, "rand"), (50000, 3000, 1, "rand"),
