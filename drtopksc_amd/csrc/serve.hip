@@ -330,11 +330,13 @@ void worker_main(drtopk_server* s, size_t si) {
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
         if (sl.busy) {
-            if (!retire(s, sl)) {
-                lk.unlock();
+            // only this worker retires its slot: wait for the launch without the lock, so that
+            // polling workers do not keep the submitting client (and each other) off it
+            lk.unlock();
+            while (hipEventQuery(sl.ev) == hipErrorNotReady)
                 std::this_thread::yield();  // a sleep would add tens of microseconds of timer slack
-                lk.lock();
-            }
+            lk.lock();
+            retire(s, sl);
             continue;
         }
         if (s->stop && s->queue.empty()) break;
