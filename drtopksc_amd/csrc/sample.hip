@@ -221,6 +221,20 @@ __device__ __forceinline__ u32x4 load4(const u32x4* p) { return __builtin_nontem
 constexpr int GROUP_BATCHES = 8;                      // 16-byte loads in flight per lane
 constexpr int GROUP_CHUNK = GROUP_BATCHES * WG * 4;   // 8192 keys per workgroup chunk
 
+// Maximum over each aligned group of g lanes (g = 1, 2, .., 64, uniform) with DPP moves, complete
+// in the group's last lane: quad permutes and the half-row / row mirrors give every lane of a
+// 16-lane row its row's maximum, row broadcast 15 then carries row 0 into row 1 and row 2 into
+// row 3 (32-lane groups), and row broadcast 31 row 1 into rows 2 and 3 (the whole wave).
+__device__ __forceinline__ u32 group_max_last(u32 m, int g) {
+    if (g >= 2) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    if (g >= 4) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    if (g >= 8) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    if (g >= 16) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x140, 0xF, 0xF, false)); // row_mirror
+    if (g >= 32) m = max(m, (u32)__builtin_amdgcn_update_dpp(0, (int)m, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    if (g >= 64) m = max(m, (u32)__builtin_amdgcn_update_dpp(0, (int)m, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return m;
+}
+
 template <class K, int BETA, int POSM, int FUSE, class LT>
 __global__ __launch_bounds__(WG) void k_sample_groups(const u32* __restrict__ keys, u64 n, int alpha, int vec_ok,
                                                       u64 nchunks, u32* __restrict__ dvals, void* __restrict__ dpos,
@@ -278,15 +292,45 @@ __global__ __launch_bounds__(WG) void k_sample_groups(const u32* __restrict__ ke
                         if (i0 + c < n) t.insert(kv[c], i0 + c);
                 }
                 const int g = 1 << (alpha - 2);  // lanes per subrange (1..64)
-                for (int off = g >> 1; off > 0; off >>= 1) t.shfl_merge(off);
                 const u64 s = i0 >> alpha;
-                const bool leader = (lane & (g - 1)) == 0 && i0 < n && s < S;
-                if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos, nullptr);
-                fuse_add<BETA, FUSE, LT>(t, leader, lhist, hshift, &mn, &mx);
+                if (BETA == 1 && POSM == 0 && full) {
+                    // one maximum per subrange: DPP merge, stored by the group's last lane
+                    LT r;
+                    r.clear();
+                    r.insert(group_max_last(t.val(0), g), 0);
+                    const bool leader = (lane & (g - 1)) == g - 1;
+                    if (leader) store_delegates<BETA, POSM, LT>(r, s, alpha, dvals, dpos, nullptr);
+                    fuse_add<BETA, FUSE, LT>(r, leader, lhist, hshift, &mn, &mx);
+                } else {
+                    for (int off = g >> 1; off > 0; off >>= 1) t.shfl_merge(off);
+                    const bool leader = (lane & (g - 1)) == 0 && i0 < n && s < S;
+                    if (leader) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos, nullptr);
+                    fuse_add<BETA, FUSE, LT>(t, leader, lhist, hshift, &mn, &mx);
+                }
             }
         }
     }
     fuse_flush<FUSE>(lhist, ghist, bins, mn, mx, ctrl, red);
+}
+
+// A whole wave's subrange s (every lane holds part of it): merge and store.  One maximum without
+// positions merges with DPP moves and is stored by lane 63; other lists merge by lane shuffles.
+template <int BETA, int POSM, int FUSE, class LT>
+__device__ __forceinline__ void wave_finish(LT& t, u64 s, int alpha, u32* dvals, void* dpos, u32* lhist, int hshift,
+                                            u32* mn, u32* mx) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    if constexpr (BETA == 1 && POSM == 0) {
+        LT r;
+        r.clear();
+        r.insert(group_max_last(t.val(0), WAVE), 0);
+        if (lane == WAVE - 1) store_delegates<BETA, POSM, LT>(r, s, alpha, dvals, dpos, nullptr);
+        fuse_add<BETA, FUSE, LT>(r, lane == WAVE - 1, lhist, hshift, mn, mx);
+    } else {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t.shfl_merge(off);
+        if (lane == 0) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos, nullptr);
+        fuse_add<BETA, FUSE, LT>(t, lane == 0, lhist, hshift, mn, mx);
+    }
 }
 
 // ---------------------------------------------------------------- wave units, alpha >= 9
@@ -326,11 +370,8 @@ __global__ __launch_bounds__(WG) void k_sample_waves(const u32* __restrict__ key
                     t.insert(K::to_ord(raw[j].z), i0 + 2);
                     t.insert(K::to_ord(raw[j].w), i0 + 3);
                 }
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) t.shfl_merge(off);
                 const u64 s = (u0 >> alpha) + r;
-                if (lane == 0) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos, nullptr);
-                fuse_add<BETA, FUSE, LT>(t, lane == 0, lhist, hshift, &mn, &mx);
+                wave_finish<BETA, POSM, FUSE, LT>(t, s, alpha, dvals, dpos, lhist, hshift, &mn, &mx);
             }
         } else if (full) {
             // one subrange per unit, ps >= 8 loads per lane in batches of 8
@@ -356,11 +397,7 @@ __global__ __launch_bounds__(WG) void k_sample_waves(const u32* __restrict__ key
                     }
                 }
             }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) t.shfl_merge(off);
-            const u64 s = u0 >> alpha;
-            if (lane == 0) store_delegates<BETA, POSM, LT>(t, s, alpha, dvals, dpos, nullptr);
-            fuse_add<BETA, FUSE, LT>(t, lane == 0, lhist, hshift, &mn, &mx);
+            wave_finish<BETA, POSM, FUSE, LT>(t, u0 >> alpha, alpha, dvals, dpos, lhist, hshift, &mn, &mx);
         } else {
             // tail unit: scalar loads, one subrange at a time
             for (int r = 0; r < spu; ++r) {

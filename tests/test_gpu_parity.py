@@ -89,6 +89,13 @@ def test_sample_delegates(orc, dt, alpha, beta):
                     gv = orc.f32_to_ordered(gv)
                 assert np.array_equal(gv, want_vals)
                 assert np.array_equal(pos.cpu().numpy().view(np.uint32), want_pos)
+                if beta == 1:  # values only: the lane-merge form without positions
+                    vals, _ = dt.sample_delegates(dev, alpha, beta, shuffle=shuffle, want_pos=False)
+                    torch.cuda.synchronize()
+                    gv = vals.cpu().numpy().view(np.uint32)
+                    if dist == 3:
+                        gv = orc.f32_to_ordered(gv)
+                    assert np.array_equal(gv, want_vals), "values without positions"
 
 
 # ------------------------------------------------------------------ full pipeline
