@@ -153,6 +153,33 @@ def test_output_subsets(orc, dt, want):
                 assert gst[s] == ost[s], (name, k, want, s, gst[s], ost[s])
 
 
+def test_f32_special_values(orc, dt):
+    """float32 keys with -0.0 / +0.0 ties, +-inf, NaNs of both signs and several payloads, and
+    denormals beside N(0,1) values: every family orders them by the sign-flipped bits (a NaN with
+    the sign bit clear above +inf, one with it set below -inf), as the oracle does."""
+    rng = np.random.default_rng(123)
+    n = (1 << 20) + 3
+    v = rng.standard_normal(n).astype(np.float32).view(np.uint32).copy()
+    kind = rng.integers(0, 100, n)
+    v[kind < 15] = 0x80000000                                   # -0.0
+    v[(kind >= 15) & (kind < 30)] = 0                           # +0.0
+    v[(kind >= 30) & (kind < 33)] = 0x7F800000                  # +inf
+    v[(kind >= 33) & (kind < 36)] = 0xFF800000                  # -inf
+    v[kind == 36] = 0x7FC00000                                  # quiet NaN
+    v[kind == 37] = 0xFFC00001                                  # negative NaN
+    v[kind == 38] = 0x7F800001                                  # signalling NaN payload
+    den = kind == 39
+    v[den] = rng.integers(1, 0x007FFFFF, int(den.sum()), dtype=np.uint64).astype(np.uint32) | \
+        (rng.integers(0, 2, int(den.sum()), dtype=np.uint64).astype(np.uint32) << 31)
+    ok = orc.f32_to_ordered(v)
+    dev = to_dev_f32(v)
+    for algo in (0, 1):
+        for k in (1, 7, 1000, 1 << 15, n // 3, n):
+            compare(orc, dt, "f32special", ok, dev, k, algo, orc.FLAGS_DEFAULT, f32=True)
+    for k in (128, 4096):
+        compare(orc, dt, "f32special", ok, dev, k, 2, orc.FLAGS_DEFAULT, alpha=6, f32=True)
+
+
 @pytest.mark.parametrize("flags", [0, 1, 2, 3, 5, 6, 7])
 @pytest.mark.parametrize("nb", [4, 8, 16])
 def test_pipeline_radix_options(orc, dt, flags, nb):
