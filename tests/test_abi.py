@@ -72,7 +72,7 @@ def test_alpha_tuned_resolves(L):
     """DRTOPK_ALPHA_TUNED (-2) takes the measured table's alpha / beta where it has a cell and the
     reference policy elsewhere; below -2 is invalid."""
     assert ws(L, 0, 1 << 30, 1 << 16, alpha=-2)[0] == 0
-    assert ws(L, 0, 1 << 20, 100, alpha=-2) == ws(L, 0, 1 << 20, 100, alpha=-1)  # no cell: the policy
+    assert ws(L, 0, 1 << 14, 100, alpha=-2) == ws(L, 0, 1 << 14, 100, alpha=-1)  # no cell below 2^16: the policy
     assert ws(L, 1, 1 << 30, 1 << 16, alpha=-2) == ws(L, 1, 1 << 30, 1 << 16, alpha=-1)  # bucket: the policy
     assert ws(L, 0, 1 << 20, 100, alpha=-3)[0] == 1
 
