@@ -143,9 +143,17 @@ namespace {
 
 bool is_solo(const drtopk_server* s, const drtopk_query& q) { return q.n >= s->cfg.solo_n || q.k > s->cfg.solo_k; }
 
+// Parameters of a single-vector launch (the server's large requests and the one-at-a-time
+// baseline): the reference configuration with the measured (alpha, beta) table, which returns
+// the same outputs as the reference policy.
+void solo_params(drtopk_params* p) {
+    drtopk_params_default(p);
+    p->alpha = DRTOPK_ALPHA_TUNED;
+}
+
 u64 solo_ws(const drtopk_query& q) {
     drtopk_params p;
-    drtopk_params_default(&p);
+    solo_params(&p);
     size_t b = 0;
     if (drtopk_workspace_size(DRTOPK_ALGO_RADIX, (drtopk_dtype)q.dtype, q.n, q.k, &p, &b) != DRTOPK_OK) return ~0ull;
     return b;
@@ -268,7 +276,7 @@ drtopk_status issue(drtopk_server* s, Slot& sl, Prep& pr) {
     drtopk_status st = DRTOPK_OK;
     if (pr.solo) {
         drtopk_params p;
-        drtopk_params_default(&p);
+        solo_params(&p);
         const drtopk_query& q = sl.solo_q;
         st = drtopk_select(DRTOPK_ALGO_RADIX, (drtopk_dtype)q.dtype, (const u32*)s->cfg.d_store + q.offset, q.n, q.k,
                            &p, A + pr.scratch_off, pr.scratch, A + kth_off, A + keys_off, (int64_t*)(A + idx_off),
@@ -570,7 +578,7 @@ drtopk_status drtopk_replay_serial(const void* d_store, uint64_t store_len, cons
     c.d_store = d_store;
     c.store_len = store_len;
     drtopk_params p;
-    drtopk_params_default(&p);
+    solo_params(&p);
     u64 ws = 256, kmax = 1;
     for (u64 i = 0; i < count; ++i) {
         drtopk_status st = check_query(c, &q[i]);

@@ -255,7 +255,8 @@ void drtopk_tuning_default(drtopk_tuning* t);
  * budget_bytes that holds every launch's scratch and the requests' outputs.  Requests name a
  * row of the key store (offset, n, k, dtype).  Whenever a stream slot is free the worker
  * launches everything queued: requests with n >= solo_n or k > solo_k as single-vector calls
- * (drtopk_select), the rest as one batched launch (drtopk_select_batched).  Requests that arrive
+ * (drtopk_select with the reference configuration and DRTOPK_ALPHA_TUNED), the rest as one
+ * batched launch (drtopk_select_batched).  Requests that arrive
  * while launches are running join the next launch as soon as a slot and arena space free up, so
  * launches of different streams overlap on the device.  Admission: a request waits in the queue
  * until its outputs and scratch fit the arena; one that could never fit is refused with
@@ -312,7 +313,8 @@ drtopk_status drtopk_server_destroy(drtopk_server* srv);
 drtopk_status drtopk_server_replay(drtopk_server* srv, const drtopk_query* q, const uint64_t* arrival_ns,
                                    uint64_t count, drtopk_result* results, uint64_t* t0_ns);
 /* The one-at-a-time baseline: the same trace served first-come first-served by single-vector
- * drtopk_select calls on one stream, each synchronised before the next starts. */
+ * drtopk_select calls (the server's parameters for them) on one stream, each synchronised before
+ * the next starts. */
 drtopk_status drtopk_replay_serial(const void* d_store, uint64_t store_len, const drtopk_query* q,
                                    const uint64_t* arrival_ns, uint64_t count, drtopk_result* results,
                                    uint64_t* t0_ns);
