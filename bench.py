@@ -340,7 +340,7 @@ def served_leg(dt, torch, nreq: int = 2000, rate: float = 20000.0):
     float32 N(0,1), k log-uniform in [1, min(n/4, 2^16)], Poisson arrivals at `rate`, keys resident
     in one device store, 64 GiB scratch arena.  The same trace is served (a) by the continuous-
     batching server and (b) one request at a time by single-vector calls; both also saturated
-    (every request arriving at t = 0)."""
+    (every request arriving at t = 0), three times each, the median run reported."""
     from drtopksc_amd.workloads import pack, serve_mix
     mix = serve_mix(seed=8, nreq=nreq, rate_per_s=rate)
     rows, keys_len, _ = pack([(n, k, d) for _, n, k, d in mix])
@@ -363,12 +363,19 @@ def served_leg(dt, torch, nreq: int = 2000, rate: float = 20000.0):
         return {"req_s": round(len(res) / span, 1), "gelem_s": round(total / span / 1e9, 2),
                 "p50_us": round(float(np.percentile(lat, 50)), 1), "p99_us": round(float(np.percentile(lat, 99)), 1)}
 
+    def median_of(runs):  # the run with the median throughput, and every run's throughput
+        runs = sorted(runs, key=lambda x: x["req_s"])
+        return dict(runs[len(runs) // 2], runs_req_s=[x["req_s"] for x in runs])
+
     with dt.Server(store) as srv:
         srv.replay(qs[:64], [0] * 64)  # warm-up: code objects, pinned staging
         r_rate, t0 = srv.replay(qs, arr)
         served_rate = summ(r_rate, t0)
-        r_sat, t1 = srv.replay(qs, sat)
-        served_sat = summ(r_sat, t1)
+        sat_runs = []
+        for _ in range(3):
+            r_sat, t1 = srv.replay(qs, sat)
+            sat_runs.append(summ(r_sat, t1))
+        served_sat = median_of(sat_runs)
     # launches in flight (worker threads, one stream each): this host gives a process 4 hardware
     # queues (GPU_MAX_HW_QUEUES), so streams beyond 4 share queues
     inflight = {}
@@ -379,8 +386,11 @@ def served_leg(dt, torch, nreq: int = 2000, rate: float = 20000.0):
             inflight[m] = summ(r_m, tm)
     s_rate, t2 = dt.replay_serial(store, qs, arr)
     serial_rate = summ(s_rate, t2)
-    s_sat, t3 = dt.replay_serial(store, qs, sat)
-    serial_sat = summ(s_sat, t3)
+    ser_runs = []
+    for _ in range(3):
+        s_sat, t3 = dt.replay_serial(store, qs, sat)
+        ser_runs.append(summ(s_sat, t3))
+    serial_sat = median_of(ser_runs)
     identical = all(r_rate[i].kth_bits == s_rate[i].kth_bits == r_sat[i].kth_bits for i in range(len(qs)))
     del store
     torch.cuda.empty_cache()

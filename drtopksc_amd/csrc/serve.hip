@@ -76,6 +76,11 @@ class Arena {
             }
         }
     }
+    u64 largest() const {  // the largest free block
+        u64 m = 0;
+        for (const auto& f : free_) m = std::max(m, f.second);
+        return m;
+    }
     u64 size() const { return size_; }
     u64 used() const { return used_; }
 
@@ -100,6 +105,7 @@ struct Req {
     u64 keys_off = 0, idx_off = 0;  // byte offsets of its outputs in the arena
     u64 block = 0;                  // OutBlock id
     u32 launch_rows = 0, solo = 0;
+    u64 need = 0;  // arena bytes of its outputs and scratch when it runs as a single-vector call
     bool released = false;
 };
 
@@ -190,22 +196,42 @@ struct Prep {
     size_t plan_bytes = 0, ws_bytes = 0;
 };
 
+// Queue entries a batch looks at past its first row: bounds the work done under the lock.
+constexpr size_t BATCH_SCAN = 16384;
+
 // Pick the next launch from the queue and reserve its arena space.  Lock held.  Returns false
 // when the queue is empty or nothing fits the arena now.
+//
+// A batch takes the small requests of the queue in arrival order, also those behind queued large
+// requests, which keeps launches few when large and small requests interleave.  A client that
+// waits and releases in submission order holds the outputs of such a batch until it has waited
+// for the large requests the batch passed, so the batch is taken only if the arena's largest free
+// block still holds the outputs and scratch of each of them afterwards; otherwise only the small
+// requests ahead of the first large one go.  Every batch that passes a request leaves it room,
+// and whatever else holds arena space then is scratch of running launches or outputs the client
+// releases before waiting for it, so the request fits once it reaches the head of the queue.
 bool prepare(drtopk_server* s, Slot& sl, Prep& pr) {
     if (s->queue.empty()) return false;
     pr = Prep();
     const u64 head = s->queue.front();
     pr.solo = is_solo(s, s->reqs[head].q);
+    std::vector<u64> passed;  // per taken request: the largest need of the large requests before it
+    size_t in_order = 1;      // taken requests ahead of the first large one
     if (pr.solo) {
         pr.take.push_back(head);
     } else {
-        // the small requests ahead of the first large one, in arrival order: a batch never
-        // overtakes a queued large request, so outputs held by a client that waits and releases
-        // in submission order cannot keep that request from ever fitting the arena
+        u64 need = 0;
+        size_t seen = 0;
         for (u64 t : s->queue) {
-            if (pr.take.size() >= s->cfg.max_batch_rows || is_solo(s, s->reqs[t].q)) break;
+            if (pr.take.size() >= s->cfg.max_batch_rows || seen++ > BATCH_SCAN) break;
+            const Req& r = s->reqs[t];
+            if (is_solo(s, r.q)) {
+                need = std::max(need, r.need);
+                continue;
+            }
             pr.take.push_back(t);
+            passed.push_back(need);
+            if (need == 0) in_order = pr.take.size();
         }
     }
     for (;;) {  // shrink the batch until outputs + scratch fit the arena
@@ -237,9 +263,16 @@ bool prepare(drtopk_server* s, Slot& sl, Prep& pr) {
             s->arena.release(pr.out_off, pr.out_bytes);
             ok = false;
         }
+        const bool passes = !pr.solo && passed[rows - 1] > 0;
+        if (ok && passes && s->arena.largest() < passed[rows - 1]) {
+            s->arena.release(pr.scratch_off, pr.scratch);
+            s->arena.release(pr.out_off, pr.out_bytes);
+            pr.take.resize(in_order);  // the passed requests would no longer fit
+            continue;
+        }
         if (ok) break;
         if (rows == 1) return false;  // wait for running launches to free the arena
-        pr.take.resize(rows / 2);
+        pr.take.resize(passes && rows / 2 < in_order ? in_order : rows / 2);
     }
     const u64 rows = pr.take.size();
     const u64 keys_off = pr.out_off + align_up(4 * rows, 256), idx_off = keys_off + align_up(4 * pr.ksum, 256);
@@ -256,9 +289,12 @@ bool prepare(drtopk_server* s, Slot& sl, Prep& pr) {
         r.solo = pr.solo ? 1u : 0u;
         kacc += r.q.k;
     }
-    std::deque<u64> rest;
-    for (u64 t : s->queue)
-        if (std::find(pr.take.begin(), pr.take.end(), t) == pr.take.end()) rest.push_back(t);
+    std::deque<u64> rest;  // the taken requests are a subsequence of the queue
+    size_t j = 0;
+    for (u64 t : s->queue) {
+        if (j < pr.take.size() && pr.take[j] == t) ++j;
+        else rest.push_back(t);
+    }
     s->queue.swap(rest);
     sl.busy = true;
     sl.tickets = pr.take;
@@ -320,7 +356,7 @@ bool retire(drtopk_server* s, Slot& sl, drtopk_status failed = DRTOPK_OK) {
     sl.busy = false;
     sl.tickets.clear();
     s->cv_done.notify_all();
-    s->cv_work.notify_all();  // arena space and a slot are free again
+    s->cv_work.notify_one();  // arena space is free again (the retiring worker prepares next)
     return true;
 }
 
@@ -363,6 +399,9 @@ void worker_main(drtopk_server* s, size_t si) {
             continue;
         }
         if (pr.solo) sl.solo_q = s->reqs[pr.take[0]].q;
+        // wakeups are chained, one idle worker at a time: waking every worker on each submit
+        // keeps the submitting thread off the lock
+        if (!s->queue.empty()) s->cv_work.notify_one();
         const u64 tl = now_ns();
         for (u64 t : pr.take) s->reqs[t].t_launch = tl;
         lk.unlock();
@@ -479,11 +518,12 @@ drtopk_status drtopk_server_submit(drtopk_server* s, const drtopk_query* q, uint
     const u64 t = s->next_ticket++;
     Req r;
     r.q = *q;
+    r.need = align_up(need, 256) + ob;
     r.t_submit = now_ns();
     s->reqs.emplace(t, r);
     s->queue.push_back(t);
     *ticket = t;
-    s->cv_work.notify_all();
+    s->cv_work.notify_one();
     return DRTOPK_OK;
 }
 
@@ -515,7 +555,7 @@ drtopk_status drtopk_server_release(drtopk_server* s, uint64_t ticket) {
     if (b != s->blocks.end() && --b->second.refs == 0) {
         s->arena.release(b->second.off, b->second.bytes);
         s->blocks.erase(b);
-        s->cv_work.notify_all();
+        s->cv_work.notify_one();
     }
     return DRTOPK_OK;
 }

@@ -86,8 +86,8 @@ def test_budget_large_request_in_order(orc, dt):
     """An arena just above one large request's need, small requests queued before and after it,
     every request waited for and released in submission order.  The rows of a batch share one
     output block, freed once all of them are released: a batch that took small requests from
-    behind the large one would keep it from ever fitting.  A finite wait turns such a stall into
-    a failure instead of a hang."""
+    behind the large one without leaving it room would keep it from ever fitting.  A finite wait
+    turns such a stall into a failure instead of a hang."""
     small_a = [(40000 + 1009 * i, 1 + 7 * i, i % 2) for i in range(8)]
     large = [(1 << 20, 3000, 0)]
     small_b = [(41000 + 1013 * i, 2 + 5 * i, (i + 1) % 2) for i in range(8)]
@@ -106,6 +106,35 @@ def test_budget_large_request_in_order(orc, dt):
             r = srv.wait(t, timeout_ns=60 * 10 ** 9)
             check_one(orc, dt, srv, store, q, o, r, "large-in-order")
             srv.release(t)
+
+
+def test_batch_takes_small_requests_behind_large(orc, dt):
+    """With room in the arena a batch takes the small requests queued behind a large one.  One
+    stream; eight 2^26-key requests keep it busy (milliseconds) while the replay queues 8 small
+    requests, a large one and 8 more small ones (microseconds), so when the stream frees up the
+    queue holds all 17: the 16 small requests go in one launch, then the large one runs alone."""
+    small_a = [(30000 + 1009 * i, 1 + 7 * i, i % 2) for i in range(8)]
+    large = [(1 << 20, 3000, 0)]
+    small_b = [(31000 + 1013 * i, 2 + 5 * i, (i + 1) % 2) for i in range(8)]
+    busy = [(1 << 26, 1000, 0)]
+    store, qs, ordered = build_store(orc, small_a + large + small_b + busy, seed0=9500)
+    seq = [qs[-1]] * 8 + qs[:-1]
+    print("[case] serve batch behind large request", flush=True)
+    with dt.Server(store, budget_bytes=1 << 32, max_inflight=1, solo_n=1 << 18, max_batch_rows=64) as srv:
+        res, _ = srv.replay(seq, [0] * len(seq))
+    rs = list(res)[8:]
+    assert all(r.status == 0 for r in res)
+    for i, (q, o, r) in enumerate(zip(qs[:-1], ordered[:-1], rs)):
+        K, _, _ = orc.topk_reference(o, q[2])
+        kk = np.array([r.kth_bits], np.uint32)
+        if q[3] == 1:
+            kk = orc.f32_to_ordered(kk)
+        assert int(kk[0]) == K, (i, q)
+        if i == 8:
+            assert r.solo == 1 and r.launch_rows == 1, (i, r.solo, r.launch_rows)
+        else:
+            assert r.solo == 0 and r.launch_rows == 16, (i, r.solo, r.launch_rows)
+    assert rs[8].t_launch_ns >= rs[9].t_launch_ns == rs[0].t_launch_ns
 
 
 def test_replay_matches_serial(orc, dt):
