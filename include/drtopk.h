@@ -256,7 +256,10 @@ void drtopk_tuning_default(drtopk_tuning* t);
  * row of the key store (offset, n, k, dtype).  Whenever a stream slot is free the worker
  * launches everything queued: requests with n >= solo_n or k > solo_k as single-vector calls
  * (drtopk_select with the reference configuration and DRTOPK_ALPHA_TUNED), the rest as one
- * batched launch (drtopk_select_batched).  Requests that arrive
+ * batched launch (drtopk_select_batched).  A batch takes small requests queued behind large ones
+ * only while the arena's largest free block still holds the outputs and scratch of every large
+ * request it passes, so a client that waits and releases in submission order never stalls one.
+ * Requests that arrive
  * while launches are running join the next launch as soon as a slot and arena space free up, so
  * launches of different streams overlap on the device.  Admission: a request waits in the queue
  * until its outputs and scratch fit the arena; one that could never fit is refused with
@@ -270,7 +273,7 @@ typedef struct drtopk_server_config {
     uint64_t budget_bytes;    /* device arena for scratch and outputs (0: 64 GiB, capped at 90% of free memory) */
     uint32_t max_inflight;    /* launches on the device at once, one stream and worker thread each (0: 8) */
     uint32_t max_batch_rows;  /* rows per batched launch (0: 4096) */
-    uint64_t solo_n;          /* n at or above which a request runs as a single-vector call (0: 2^22) */
+    uint64_t solo_n;          /* n at or above which a request runs as a single-vector call (0: 2^23) */
     uint64_t solo_k;          /* k above which a request runs as a single-vector call (0: 4096) */
     const drtopk_tuning* tuning; /* batched launches' tuning table (NULL: built-in) */
     uint32_t reserved[4];     /* must be zero */
