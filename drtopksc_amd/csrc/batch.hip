@@ -261,6 +261,15 @@ struct SrcG {  // global ordered u32 array (delegates), 16-byte aligned
         for (int j = 0; j < 4; ++j) ix[j] = (u32)(e + j);
         return v;
     }
+    // slots [0, bulk_end(E)) can be read with load4_fast: no per-lane bounds test
+    __device__ __forceinline__ u64 bulk_end(u64 E, u64 step) const { return min(E, m) / step * step; }
+    __device__ __forceinline__ u32 load4_fast(u64 e, u32* x, u32* ix) const {
+        const u32x4 q = *(const u32x4*)(p + e);
+        x[0] = q[0]; x[1] = q[1]; x[2] = q[2]; x[3] = q[3];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ix[j] = (u32)(e + j);
+        return 0xF;
+    }
 };
 
 struct SrcL {  // LDS keys + row indices, 16-byte aligned
@@ -285,6 +294,14 @@ struct SrcL {  // LDS keys + row indices, 16-byte aligned
             }
         }
         return v;
+    }
+    __device__ __forceinline__ u64 bulk_end(u64 E, u64 step) const { return min(E, (u64)m) / step * step; }
+    __device__ __forceinline__ u32 load4_fast(u64 e, u32* x, u32* ix) const {
+        const u32x4 q = *(const u32x4*)(key + e);
+        const u32x4 r = *(const u32x4*)(idx + e);
+        x[0] = q[0]; x[1] = q[1]; x[2] = q[2]; x[3] = q[3];
+        ix[0] = r[0]; ix[1] = r[1]; ix[2] = r[2]; ix[3] = r[3];
+        return 0xF;
     }
 };
 
@@ -335,7 +352,43 @@ struct SrcV {
         }
         return v;
     }
+    // alpha == 1 reads slot by slot and rows under 4 keys have no in-bounds granule to clamp
+    // to: those take load4 throughout
+    __device__ __forceinline__ u64 bulk_end(u64 E_, u64 step) const {
+        return (alpha == 1 || n < 4) ? 0 : min(E_, E) / step * step;
+    }
+    // One 16-byte load from an address clamped into the row, so that no lane's load depends on a
+    // per-lane condition (such a load is waited for where the branches join); only a granule
+    // that runs past the row's end (its last subrange) is re-read key by key.
+    __device__ __forceinline__ u32 load4_fast(u64 e, u32* x, u32* ix) const {
+        const u64 base = alpha == 0 ? e : (((u64)sel[e >> alpha] << alpha) | (e & ((1ull << alpha) - 1)));
+        const u64 bc = base + 4 <= n ? base : ((n - 4) & ~3ull);
+        const u32x4 q = *(const u32x4*)(row + bc);
+        x[0] = q[0]; x[1] = q[1]; x[2] = q[2]; x[3] = q[3];
+        u32 v = 0xF;
+        if (bc != base) {
+            v = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool ok = base + j < n;
+                x[j] = ok ? row[base + j] : 0u;
+                v |= (ok ? 1u : 0u) << j;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            ix[j] = (u32)(base + j);
+            x[j] = cvt(x[j]);
+            if (x[j] < lo) v &= ~(1u << j);
+        }
+        return v;
+    }
 };
+
+// Granules (4 slots) per thread in each step of the bulk loops below: BR_U 16-byte loads in
+// flight per lane, and one block scan per BR_NT * 4 * BR_U slots in the ordered pass.
+constexpr int BR_U = 4;
+constexpr u64 BR_STEP = (u64)BR_NT * 4 * BR_U;
 
 struct RowLds {
     u32 hist[BR_W * 256];
@@ -343,12 +396,27 @@ struct RowLds {
     u32 misc[16];
 };
 
+// One granule of a digit pass: its valid keys under the prefix counted in the wave's histogram
+// (and, with gather, listed as candidates).
+__device__ __forceinline__ void hist_step(const u32 (&x)[4], u32 v, u32 mask, u32 prefix, int shift, int R, u32* wh,
+                                          bool gather, u32* cand, u32 cap, RowLds& L) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const bool act = ((v >> j) & 1u) && (x[j] & mask) == prefix;
+        wave_hist_add(wh, act, (x[j] >> shift) & (R - 1));
+        if (gather && act) {
+            const u32 p = atomicAdd(&L.misc[0], 1u);
+            if (p < cap) cand[p] = x[j];
+        }
+    }
+}
+
 // k-th largest (ascending rank m-k) of the valid slots of src by radix digits of nb bits,
 // most significant first.  With `gather`, once the candidates of the current prefix fit in
 // cand[0..cap) the next pass copies them there and the remaining digits run on LDS.
 // Returns the key; *gt = valid slots strictly greater.
 template <class Src>
-__device__ u32 wg_kth(const Src& src, u64 E, u64 k, int nb, bool gather_ok, u32* cand, u32 cap, RowLds& L,
+__device__ __forceinline__ u32 wg_kth(const Src& src, u64 E, u64 k, int nb, bool gather_ok, u32* cand, u32 cap, RowLds& L,
                       u64* gt_out) {
     const int t = threadIdx.x, w = t / WAVE, lane = t & (WAVE - 1);
     const int R = 1 << nb;
@@ -368,19 +436,19 @@ __device__ u32 wg_kth(const Src& src, u64 E, u64 k, int nb, bool gather_ok, u32*
                 wave_hist_add(wh, i < mc && (x & mask) == prefix, (x >> shift) & (R - 1));
             }
         } else {
-            for (u64 e0 = 0; e0 < E; e0 += (u64)BR_NT * 4) {
+            const u64 Eb = src.bulk_end(E, BR_STEP);
+            for (u64 e0 = 0; e0 < Eb; e0 += BR_STEP) {  // whole steps: every load issued before use
+                u32 x[BR_U][4], ix[BR_U][4], v[BR_U];
+#pragma unroll
+                for (int u = 0; u < BR_U; ++u) v[u] = src.load4_fast(e0 + ((u64)u * BR_NT + t) * 4, x[u], ix[u]);
+#pragma unroll
+                for (int u = 0; u < BR_U; ++u) hist_step(x[u], v[u], mask, prefix, shift, R, wh, gather, cand, cap, L);
+            }
+            for (u64 e0 = Eb; e0 < E; e0 += (u64)BR_NT * 4) {
                 const u64 e = e0 + (u64)t * 4;
                 u32 x[4], ix[4];
                 const u32 v = e < E ? src.load4(e, x, ix) : 0u;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const bool act = ((v >> j) & 1u) && (x[j] & mask) == prefix;
-                    wave_hist_add(wh, act, (x[j] >> shift) & (R - 1));
-                    if (gather && act) {
-                        const u32 p = atomicAdd(&L.misc[0], 1u);
-                        if (p < cap) cand[p] = x[j];
-                    }
-                }
+                hist_step(x, v, mask, prefix, shift, R, wh, gather, cand, cap, L);
             }
         }
         __syncthreads();
@@ -442,37 +510,56 @@ __device__ u32 wg_kth(const Src& src, u64 E, u64 k, int nb, bool gather_ok, u32*
 // Ordered pass: calls emit(pos, key, index) for every valid slot with key > K and for the
 // first `ties` valid slots with key == K (ALL: every valid slot), pos = rank among those
 // emitted, in slot order.  Returns the number of valid slots > K (ALL: valid slots).
-template <bool ALL, class Src, class F>
-__device__ u64 wg_ordered(const Src& src, u64 E, u32 K, u64 ties, RowLds& L, F emit) {
-    const int t = threadIdx.x;
-    u64 gt_run = 0, eq_run = 0;
-    for (u64 e0 = 0; e0 < E; e0 += (u64)BR_NT * 4) {
-        const u64 e = e0 + (u64)t * 4;
-        u32 x[4], ix[4];
-        const u32 v = e < E ? src.load4(e, x, ix) : 0u;
-        u32 g = 0, q = 0;
+// One step of wg_ordered: thread t holds granules t*U .. t*U+U-1 of the step, so one block scan
+// ranks all of them in slot order.
+template <bool ALL, int U, class F>
+__device__ __forceinline__ void ordered_step(const u32 (&x)[U][4], const u32 (&ix)[U][4], const u32 (&v)[U], u32 K,
+                                             u64 ties, RowLds& L, u64& gt_run, u64& eq_run, F& emit) {
+    u32 g = 0, q = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const bool ok = (v >> j) & 1u;
-            g += (ok && (ALL || x[j] > K)) ? 1u : 0u;
-            q += (!ALL && ok && x[j] == K) ? 1u : 0u;
+            const bool ok = (v[u] >> j) & 1u;
+            g += (ok && (ALL || x[u][j] > K)) ? 1u : 0u;
+            q += (!ALL && ok && x[u][j] == K) ? 1u : 0u;
         }
-        u64 tot;
-        const u64 ex = block_excl_scan_u64<BR_NT>(((u64)q << 32) | g, L.scan, &tot);
-        u64 gb = gt_run + (u32)ex, qb = eq_run + (ex >> 32);
+    u64 tot;
+    const u64 ex = block_excl_scan_u64<BR_NT>(((u64)q << 32) | g, L.scan, &tot);
+    u64 gb = gt_run + (u32)ex, qb = eq_run + (ex >> 32);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            if (!((v >> j) & 1u)) continue;
-            if (ALL || x[j] > K) {
-                emit(gb + min(qb, ties), x[j], ix[j]);
+            if (!((v[u] >> j) & 1u)) continue;
+            if (ALL || x[u][j] > K) {
+                emit(gb + min(qb, ties), x[u][j], ix[u][j]);
                 ++gb;
-            } else if (x[j] == K) {
-                if (qb < ties) emit(gb + qb, x[j], ix[j]);
+            } else if (x[u][j] == K) {
+                if (qb < ties) emit(gb + qb, x[u][j], ix[u][j]);
                 ++qb;
             }
         }
-        gt_run += (u32)tot;
-        eq_run += tot >> 32;
+    gt_run += (u32)tot;
+    eq_run += tot >> 32;
+}
+
+template <bool ALL, class Src, class F>
+__device__ __forceinline__ u64 wg_ordered(const Src& src, u64 E, u32 K, u64 ties, RowLds& L, F emit) {
+    const int t = threadIdx.x;
+    u64 gt_run = 0, eq_run = 0;
+    const u64 Eb = src.bulk_end(E, BR_STEP);
+    for (u64 e0 = 0; e0 < Eb; e0 += BR_STEP) {
+        u32 x[BR_U][4], ix[BR_U][4], v[BR_U];
+#pragma unroll
+        for (int u = 0; u < BR_U; ++u) v[u] = src.load4_fast(e0 + ((u64)t * BR_U + u) * 4, x[u], ix[u]);
+        ordered_step<ALL, BR_U>(x, ix, v, K, ties, L, gt_run, eq_run, emit);
+    }
+    for (u64 e0 = Eb; e0 < E; e0 += (u64)BR_NT * 4) {
+        const u64 e = e0 + (u64)t * 4;
+        u32 x[1][4], ix[1][4], v[1];
+        v[0] = e < E ? src.load4(e, x[0], ix[0]) : 0u;
+        ordered_step<ALL, 1>(x, ix, v, K, ties, L, gt_run, eq_run, emit);
     }
     return gt_run;
 }
