@@ -61,7 +61,9 @@ struct BatchRow {
     u64 n, k;
     u64 out_off;    // first output slot
     u64 deleg_off;  // delegates in the workspace delegate array (multiple of 4)
-    u64 sel_off;    // selected subrange ids in the workspace id array
+    u64 sel_off;    // selected subrange ids in the workspace id array, then (when k << alpha
+                    // exceeds the LDS concatenation cache) room for k << alpha concatenated keys
+                    // and as many row indices
     u32 S;          // subranges (0 when alpha == 0)
     u32 chunk_begin, nchunks;
     u8 alpha, f32, nb, tile_log2;
@@ -272,10 +274,10 @@ struct SrcG {  // global ordered u32 array (delegates), 16-byte aligned
     }
 };
 
-struct SrcL {  // LDS keys + row indices, 16-byte aligned
+struct SrcL {  // keys + row indices (LDS, or a row's concatenation in the workspace), 16-byte aligned
     const u32* key;
     const u32* idx;
-    u32 m;
+    u64 m;
     __device__ __forceinline__ u32 load4(u64 e, u32* x, u32* ix) const {
         u32 v = 0;
         if (e + 4 <= m) {
@@ -295,7 +297,7 @@ struct SrcL {  // LDS keys + row indices, 16-byte aligned
         }
         return v;
     }
-    __device__ __forceinline__ u64 bulk_end(u64 E, u64 step) const { return min(E, (u64)m) / step * step; }
+    __device__ __forceinline__ u64 bulk_end(u64 E, u64 step) const { return min(E, m) / step * step; }
     __device__ __forceinline__ u32 load4_fast(u64 e, u32* x, u32* ix) const {
         const u32x4 q = *(const u32x4*)(key + e);
         const u32x4 r = *(const u32x4*)(idx + e);
@@ -564,10 +566,96 @@ __device__ __forceinline__ u64 wg_ordered(const Src& src, u64 E, u32 K, u64 ties
     return gt_run;
 }
 
-__global__ __launch_bounds__(BR_NT) void k_brows(const unsigned char* __restrict__ plan, const u32* __restrict__ keys,
-                                                 BatchWs* wsh, const u32* __restrict__ deleg,
-                                                 u32* __restrict__ selws, u32* __restrict__ kth_out,
-                                                 u32* __restrict__ out_keys, i64* __restrict__ out_idx) {
+// One row of k_brows (its index q in the processing order).  SPILL (alpha > 0 and k larger than
+// the LDS concatenation cache, so the filtered concatenation of C >= k keys cannot fit it): the
+// keys that overflow the cache go to the row's workspace region after its selected ids, then the
+// cached part, and the second top-k streams those C keys instead of gathering the k << alpha
+// slots through the id list twice more.  Rows with k within the cache keep the LDS path (and,
+// when the concatenation does overflow, the re-gather), without the extra code in their loops.
+template <bool SPILL>
+__device__ __forceinline__ void brow(const BatchRow* rp, u32 q, const u32* __restrict__ keys, BatchWs* wsh,
+                                     const u32* __restrict__ deleg, u32* __restrict__ selws,
+                                     u32* __restrict__ kth_out, u32* __restrict__ out_keys, i64* __restrict__ out_idx,
+                                     RowLds& L, u32* reg) {
+    const int t = threadIdx.x;
+    const u64 n = rp->n, k = rp->k, out_off = rp->out_off;
+    const int a = rp->alpha, nb = rp->nb, f32 = rp->f32;
+    const u32* row = keys + rp->key_off;
+    u32* sel = selws + rp->sel_off;
+
+    // first top-k over the delegates and range selection
+    u32 T = 0;
+    u64 E = n;
+    if (a > 0) {
+        const SrcG dg{deleg + rp->deleg_off, rp->S};
+        u64 g = 0;
+        T = wg_kth(dg, rp->S, k, nb, true, reg, REG_WORDS, L, &g);
+        const u64 ties = k - g;
+        wg_ordered<false>(dg, rp->S, T, ties, L, [&](u64 p, u32, u32 id) {
+            if (p < k) sel[p] = id;
+        });
+        __syncthreads();
+        E = k << a;
+    }
+    // filtered concatenation, cached in LDS when it fits
+    const SrcV vs{row, n, sel, a, f32, T, E};
+    u32* ckey = reg;
+    u32* cidx = reg + CONCAT_CAP;
+    u32* gck = sel + ((k + 3) & ~3ull);  // SPILL: room for E keys, then E indices
+    u32* gci = gck + ((E + 3) & ~3ull);
+    const u64 C = wg_ordered<true>(vs, E, 0u, 0, L, [&](u64 p, u32 x, u32 ix) {
+        if (p < CONCAT_CAP) {
+            ckey[p] = x;
+            cidx[p] = ix;
+        } else if (SPILL) {  // p < C <= E
+            gck[p] = x;
+            gci[p] = ix;
+        }
+    });
+    __syncthreads();
+    // every key of the top-k set is >= T, so the concatenation holds at least k keys
+    if (C < k) {
+        if (t == 0) ws_fail(&wsh->head, ERR_CONCAT_SHORT);
+        return;
+    }
+    // second top-k and extraction
+    u64 gt = 0;
+    u32 K;
+    auto emit_out = [&](u64 p, u32 x, u32 ix) {
+        if (p < k) {
+            if (out_keys) out_keys[out_off + p] = f32 ? KeyF32::from_ord(x) : x;
+            if (out_idx) out_idx[out_off + p] = (i64)ix;
+        }
+    };
+    if (C <= CONCAT_CAP) {
+        const SrcL ls{ckey, cidx, C};
+        K = wg_kth(ls, C, k, nb, false, nullptr, 0, L, &gt);
+        wg_ordered<false>(ls, C, K, k - gt, L, emit_out);
+    } else if (SPILL) {
+        for (u32 i = t; i < CONCAT_CAP; i += BR_NT) {
+            gck[i] = ckey[i];
+            gci[i] = cidx[i];
+        }
+        __syncthreads();  // the LDS cache is free again: it takes the digit passes' candidates
+        const SrcL gs{gck, gci, C};
+        K = wg_kth(gs, C, k, nb, true, reg, REG_WORDS, L, &gt);
+        wg_ordered<false>(gs, C, K, k - gt, L, emit_out);
+    } else {
+        K = wg_kth(vs, E, k, nb, true, reg, REG_WORDS, L, &gt);
+        wg_ordered<false>(vs, E, K, k - gt, L, emit_out);
+    }
+    if (t == 0 && kth_out) kth_out[q] = f32 ? KeyF32::from_ord(K) : K;
+}
+
+// The plan's rule for a row's concatenation region (it reserves the room).
+__host__ __device__ __forceinline__ bool row_spills(int alpha, u64 k) { return alpha > 0 && k > CONCAT_CAP; }
+
+// Four waves per SIMD (two workgroups per CU): the row body needs a few more registers than that
+// allows unbounded, and halving the resident rows costs more than the bound does.
+__global__ __launch_bounds__(BR_NT, 4) void k_brows(const unsigned char* __restrict__ plan, const u32* __restrict__ keys,
+                                                    BatchWs* wsh, const u32* __restrict__ deleg,
+                                                    u32* __restrict__ selws, u32* __restrict__ kth_out,
+                                                    u32* __restrict__ out_keys, i64* __restrict__ out_idx) {
     const BatchHeader* h = (const BatchHeader*)plan;
     const BatchRow* rows = (const BatchRow*)(plan + h->o_rows);
     const u32* order = (const u32*)(plan + h->o_order);
@@ -582,59 +670,10 @@ __global__ __launch_bounds__(BR_NT) void k_brows(const unsigned char* __restrict
         const u32 q = L.misc[15];
         if (q >= nrows) break;
         const BatchRow* rp = rows + order[q];
-        const u64 n = rp->n, k = rp->k, out_off = rp->out_off;
-        const int a = rp->alpha, nb = rp->nb, f32 = rp->f32;
-        const u32* row = keys + rp->key_off;
-        u32* sel = selws + rp->sel_off;
-
-        // first top-k over the delegates and range selection
-        u32 T = 0;
-        u64 E = n;
-        if (a > 0) {
-            const SrcG dg{deleg + rp->deleg_off, rp->S};
-            u64 g = 0;
-            T = wg_kth(dg, rp->S, k, nb, true, reg, REG_WORDS, L, &g);
-            const u64 ties = k - g;
-            wg_ordered<false>(dg, rp->S, T, ties, L, [&](u64 p, u32, u32 id) {
-                if (p < k) sel[p] = id;
-            });
-            __syncthreads();
-            E = k << a;
-        }
-        // filtered concatenation, cached in LDS when it fits
-        const SrcV vs{row, n, sel, a, f32, T, E};
-        u32* ckey = reg;
-        u32* cidx = reg + CONCAT_CAP;
-        const u64 C = wg_ordered<true>(vs, E, 0u, 0, L, [&](u64 p, u32 x, u32 ix) {
-            if (p < CONCAT_CAP) {
-                ckey[p] = x;
-                cidx[p] = ix;
-            }
-        });
-        __syncthreads();
-        // every key of the top-k set is >= T, so the concatenation holds at least k keys
-        if (C < k) {
-            if (t == 0) ws_fail(&wsh->head, ERR_CONCAT_SHORT);
-            continue;
-        }
-        // second top-k and extraction
-        u64 gt = 0;
-        u32 K;
-        auto emit_out = [&](u64 p, u32 x, u32 ix) {
-            if (p < k) {
-                if (out_keys) out_keys[out_off + p] = f32 ? KeyF32::from_ord(x) : x;
-                if (out_idx) out_idx[out_off + p] = (i64)ix;
-            }
-        };
-        if (C <= CONCAT_CAP) {
-            const SrcL ls{ckey, cidx, (u32)C};
-            K = wg_kth(ls, C, k, nb, false, nullptr, 0, L, &gt);
-            wg_ordered<false>(ls, C, K, k - gt, L, emit_out);
-        } else {
-            K = wg_kth(vs, E, k, nb, true, reg, REG_WORDS, L, &gt);
-            wg_ordered<false>(vs, E, K, k - gt, L, emit_out);
-        }
-        if (t == 0 && kth_out) kth_out[order[q]] = f32 ? KeyF32::from_ord(K) : K;
+        if (row_spills(rp->alpha, rp->k))
+            brow<true>(rp, order[q], keys, wsh, deleg, selws, kth_out, out_keys, out_idx, L, reg);
+        else
+            brow<false>(rp, order[q], keys, wsh, deleg, selws, kth_out, out_keys, out_idx, L, reg);
     }
 }
 
@@ -746,6 +785,7 @@ drtopk_status drtopk_batch_plan(const drtopk_row* rows, uint64_t nrows, uint64_t
             nch += ch;
             dl += align_up(S, 4);
             sl += align_up(in.k, 4);
+            if (row_spills(a, in.k)) sl += 2 * align_up(in.k << a, 4);
             cost[r] = 4 * S + (in.k << a);
         } else {
             cost[r] = 4 * in.n;

@@ -85,7 +85,10 @@ def test_batch_edges(orc, dt):
     specs = [(1, 1, 0, "rand"), (5, 5, 1, "rand"), (7, 3, 0, "rand"), (4096, 4096, 0, "rand"),
              (4099, 1, 1, "rand"), (70001, 1000, 0, "equal"), (70001, 1000, 1, "equal"),
              (300001, 4096, 0, "ties"), (300001, 777, 1, "ties"), (1 << 20, 1 << 16, 0, "rand"),
-             (1 << 20, 20000, 1, "rand"), (100003, 25000, 0, "rand"), (2 ** 21 + 3, 4096, 1, "rand")]
+             (1 << 20, 20000, 1, "rand"), (100003, 25000, 0, "rand"), (2 ** 21 + 3, 4096, 1, "rand"),
+             # k above the row phase's LDS concatenation cache (6144 keys): the concatenation goes
+             # through the row's workspace region, here with ties and equal keys filling it
+             (300001, 9000, 0, "ties"), (300003, 7000, 1, "ties"), (70001, 8000, 1, "equal")]
     rows, buf, kl, ol, ordered = make_rows(orc, specs)
     print(f"[case] batch edges rows={len(rows)}", flush=True)
     _, _, kth, tk, ti = run_batch(dt, rows, buf, kl, ol)
@@ -99,7 +102,8 @@ def test_batch_tuning_overrides(orc, dt, alpha, nb, tile):
     """Every alpha / digit width / tile the planner accepts gives the same rows."""
     specs = [(n, k, d, kind) for n, k, d, kind in [(1 << 16, 100, 0, "rand"), (50000, 3000, 1, "rand"),
                                                    (200003, 64, 0, "ties"), (4096, 500, 1, "rand"),
-                                                   (1 << 18, 1, 0, "rand"), (33333, 2048, 1, "equal")]]
+                                                   (1 << 18, 1, 0, "rand"), (33333, 2048, 1, "equal"),
+                                                   (1 << 18, 7000, 0, "rand")]]
     rows, buf, kl, ol, ordered = make_rows(orc, specs, seed0=2000 + alpha)
     t = dt.tuning_default()
     for d in range(2):
