@@ -518,7 +518,7 @@ def main():
     # HBM bytes per launch of this kernel from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the
     # same workload, FETCH_SIZE scaled by 2.000 and WRITE_SIZE by 1.000 (the factors a 4 GiB
     # device-to-device copy reads back): profiles/delegate_pass_2p30_k1024.md
-    traffic = (2 * 2_097_194 + 4_397) * 1024 if args.log2n == 30 else None
+    traffic = (2 * 2_097_212 + 2_251) * 1024 if args.log2n == 30 else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "delegate pass (k_sample_waves, alpha=11)", "bytes_per_launch": samp_bytes,
