@@ -110,19 +110,21 @@ def test_budget_large_request_in_order(orc, dt):
 
 def test_batch_takes_small_requests_behind_large(orc, dt):
     """With room in the arena a batch takes the small requests queued behind a large one.  One
-    stream; eight 2^26-key requests keep it busy (milliseconds) while the replay queues 8 small
-    requests, a large one and 8 more small ones (microseconds), so when the stream frees up the
-    queue holds all 17: the 16 small requests go in one launch, then the large one runs alone."""
+    stream; 24 single-vector requests of 2^26 keys keep it busy (milliseconds) while the replay
+    queues 8 small requests, a large one and 8 more small ones (microseconds), so when the stream
+    frees up the queue holds all 17: the 16 small requests go in one launch, then the large one
+    runs alone."""
     small_a = [(30000 + 1009 * i, 1 + 7 * i, i % 2) for i in range(8)]
     large = [(1 << 20, 3000, 0)]
     small_b = [(31000 + 1013 * i, 2 + 5 * i, (i + 1) % 2) for i in range(8)]
     busy = [(1 << 26, 1000, 0)]
     store, qs, ordered = build_store(orc, small_a + large + small_b + busy, seed0=9500)
-    seq = [qs[-1]] * 8 + qs[:-1]
+    nbusy = 24
+    seq = [qs[-1]] * nbusy + qs[:-1]
     print("[case] serve batch behind large request", flush=True)
     with dt.Server(store, budget_bytes=1 << 32, max_inflight=1, solo_n=1 << 18, max_batch_rows=64) as srv:
         res, _ = srv.replay(seq, [0] * len(seq))
-    rs = list(res)[8:]
+    rs = list(res)[nbusy:]
     assert all(r.status == 0 for r in res)
     for i, (q, o, r) in enumerate(zip(qs[:-1], ordered[:-1], rs)):
         K, _, _ = orc.topk_reference(o, q[2])
