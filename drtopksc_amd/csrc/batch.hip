@@ -102,6 +102,18 @@ __device__ __forceinline__ u32 group_max(u32 m, int g) {
     return m;
 }
 
+// The same maximum complete only in each group's last lane, with DPP moves alone (row broadcasts
+// 15 and 31 in place of the LDS permutes for 32- and 64-lane groups).
+__device__ __forceinline__ u32 group_max_last(u32 m, int g) {
+    if (g >= 2) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    if (g >= 4) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    if (g >= 8) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    if (g >= 16) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x140, 0xF, 0xF, false)); // row_mirror
+    if (g >= 32) m = max(m, (u32)__builtin_amdgcn_update_dpp(0, (int)m, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    if (g >= 64) m = max(m, (u32)__builtin_amdgcn_update_dpp(0, (int)m, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return m;
+}
+
 __device__ __forceinline__ u32 max4(const u32x4& v) { return max(max(v[0], v[1]), max(v[2], v[3])); }
 
 constexpr u32 BS_WKEYS = BS_UNR * WAVE * 4;  // keys per wave per batch (2048)
@@ -158,8 +170,9 @@ __device__ __forceinline__ void bsample_tile(const u32* __restrict__ src, u64 le
 #pragma unroll
             for (int u = 0; u < BS_UNR; ++u) {
                 const u64 e = wb + (u64)(u * WAVE + lane) * 4;
-                const u32 m = group_max(max4(q[u]), g);
-                if ((lane & (g - 1)) == 0 && e < len) dout[e >> a] = m;
+                const u32 m = group_max_last(max4(q[u]), g);
+                const u64 ef = e - 4ull * (u64)(lane & (g - 1));  // the group's (subrange's) first key
+                if ((lane & (g - 1)) == g - 1 && ef < len) dout[ef >> a] = m;
             }
         } else if (a <= 11) {
             const int per = 1 << (a - 8);  // loads per subrange
@@ -169,8 +182,8 @@ __device__ __forceinline__ void bsample_tile(const u32* __restrict__ src, u64 le
                 acc = max(acc, max4(q[u]));
                 if (((u + 1) & (per - 1)) == 0) {
                     const u64 e0 = wb + (u64)(u + 1 - per) * WAVE * 4;  // the subrange's first key
-                    const u32 m = group_max(acc, WAVE);
-                    if (lane == 0 && e0 < len) dout[e0 >> a] = m;
+                    const u32 m = group_max_last(acc, WAVE);
+                    if (lane == WAVE - 1 && e0 < len) dout[e0 >> a] = m;
                     acc = 0;
                 }
             }
