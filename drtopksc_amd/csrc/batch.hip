@@ -89,21 +89,11 @@ __device__ __forceinline__ u32 ord(u32 b) {
     return F32 ? KeyF32::to_ord(b) : b;
 }
 
-// Maximum over each aligned group of g lanes (g = 1, 2, 4, .., 64, uniform over the wave); every
-// lane of a group ends with its group's maximum.  DPP moves inside rows of 16 lanes (quad
-// permutes, then the half-row and row mirrors), LDS permutes across rows.
-__device__ __forceinline__ u32 group_max(u32 m, int g) {
-    if (g >= 2) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
-    if (g >= 4) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
-    if (g >= 8) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x141, 0xF, 0xF, false));  // row_half_mirror
-    if (g >= 16) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x140, 0xF, 0xF, false)); // row_mirror
-    if (g >= 32) m = max(m, (u32)__shfl_xor((int)m, 16, WAVE));
-    if (g >= 64) m = max(m, (u32)__shfl_xor((int)m, 32, WAVE));
-    return m;
-}
-
-// The same maximum complete only in each group's last lane, with DPP moves alone (row broadcasts
-// 15 and 31 in place of the LDS permutes for 32- and 64-lane groups).
+// Maximum over each aligned group of g lanes (g = 1, 2, 4, .., 64, uniform over the wave),
+// complete in the group's last lane, with DPP moves alone: quad permutes and the half-row / row
+// mirrors give every lane of a 16-lane row its row's maximum, row broadcast 15 then carries row 0
+// into row 1 and row 2 into row 3 (32-lane groups), and row broadcast 31 row 1 into rows 2 and 3
+// (the whole wave).
 __device__ __forceinline__ u32 group_max_last(u32 m, int g) {
     if (g >= 2) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
     if (g >= 4) m = max(m, (u32)__builtin_amdgcn_mov_dpp((int)m, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
@@ -191,8 +181,8 @@ __device__ __forceinline__ void bsample_tile(const u32* __restrict__ src, u64 le
             u32 acc = 0;
 #pragma unroll
             for (int u = 0; u < BS_UNR; ++u) acc = max(acc, max4(q[u]));
-            acc = group_max(acc, WAVE);
-            if (lane == 0) part[wb / BS_WKEYS] = acc;  // the wave's 2048-key maximum
+            acc = group_max_last(acc, WAVE);
+            if (lane == WAVE - 1) part[wb / BS_WKEYS] = acc;  // the wave's 2048-key maximum
         }
     }
     if (a >= 12) {
