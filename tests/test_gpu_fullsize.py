@@ -1,6 +1,7 @@
 """Full-size GPU parity: the single-vector path at |V| = 2^30 for every distribution of the
 single-GPU workload (uniform, normal(2^31, 2^28), log-normal(20, 2)), radix and bucket families,
-and a ragged |V| = 2^31 - 5 near the largest supported length, on the default and beta paths.
+a ragged |V| = 2^31 - 5 near the largest supported length, on the default and beta paths, and
+2^28 float32 N(0,1) keys.
 
 Each case is checked two ways on the host: against the OpenMP nth_element oracle
 (oracle.cpu_baseline: k-th key, top-k keys and indices under the lowest-index tie rule) and with
@@ -124,5 +125,33 @@ def test_ragged_near_limit(orc, dt):
         if k in (1 << 10, 1 << 24) and algo == 0:
             _, K, keys, idx = orc.cpu_baseline(v, k, threads=0)
             assert gk == K and np.array_equal(gidx, idx) and np.array_equal(gkeys, keys), k
+    del dev, ws
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.timeout(600)
+def test_f32_2p28(orc, dt):
+    """2^28 float32 N(0,1) keys (the float workload's size), radix and bucket, k = 2^0 .. 2^20,
+    against nth_element on the sign-flip-ordered keys.  At this size the alpha >= 9 delegate pass
+    streams four rounds of wave units per workgroup step, so its float32 instantiation is pinned
+    here as the uint32 one is at 2^30."""
+    n = 1 << 28
+    v = orc.generate(3, 5, n)
+    o = orc.f32_to_ordered(v)
+    dev = torch.from_numpy(np.ascontiguousarray(v).view(np.float32)).cuda()
+    ks = (1, 1 << 10, 1 << 16, 1 << 20)
+    b = max(dt.workspace_size(n, k, a, 1) for k in ks for a in (0, 1))
+    ws = torch.empty(b, dtype=torch.uint8, device="cuda")
+    for k in ks:
+        _, K, keys, idx = orc.cpu_baseline(o, k, threads=0)
+        for algo in (0, 1):
+            print(f"[case] full-size f32 normal n=2^28 k={k} algo={algo}", flush=True)
+            kth, tk, ti, st = dt.select(dev, k, algo=algo, stats=True, workspace=ws)
+            torch.cuda.synchronize()
+            gk = orc.f32_to_ordered(kth.cpu().numpy().view(np.uint32)[:1])[0]
+            gkeys = orc.f32_to_ordered(tk.cpu().numpy().view(np.uint32))
+            assert int(gk) == K, ("f32", k, algo, "kth")
+            assert np.array_equal(ti.cpu().numpy(), idx), ("f32", k, algo, "idx")
+            assert np.array_equal(gkeys, keys), ("f32", k, algo, "keys")
     del dev, ws
     torch.cuda.empty_cache()
