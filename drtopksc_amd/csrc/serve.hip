@@ -121,6 +121,7 @@ struct Slot {
     size_t h_kth_cap = 0;
     u32* h_err = nullptr;                     // pinned device-side error bits of the launch
     drtopk_query solo_q{};                    // the request of a single-vector launch
+    std::vector<void*> retired;               // outgrown pinned buffers, freed by destroy
 };
 
 }  // namespace
@@ -166,15 +167,18 @@ u64 solo_ws(const drtopk_query& q) {
 }
 u64 out_bytes(u64 rows, u64 ksum) { return align_up(4 * rows, 256) + align_up(4 * ksum, 256) + 8 * ksum; }
 
-// Grow a pinned host buffer (only while its slot is idle).
+// Grow a pinned host buffer (only while its slot is idle).  The outgrown buffer is kept until
+// destroy: hipHostFree synchronises the whole device, and this runs under the server lock, so
+// freeing here would hold every worker until all running launches had finished.  Sizes double,
+// so the kept buffers add up to less than the current one.
 template <class T>
-bool ensure_pinned(T** p, size_t* cap, size_t bytes) {
+bool ensure_pinned(T** p, size_t* cap, size_t bytes, std::vector<void*>& retired) {
     if (*cap >= bytes) return true;
-    if (*p) (void)hipHostFree(*p);
-    *p = nullptr;
-    *cap = 0;
+    T* np = nullptr;
     const size_t nb = std::max<size_t>(bytes, 4096) * 2;
-    if (hipHostMalloc((void**)p, nb, hipHostMallocDefault) != hipSuccess) return false;
+    if (hipHostMalloc((void**)&np, nb, hipHostMallocDefault) != hipSuccess) return false;
+    if (*p) retired.push_back(*p);
+    *p = np;
     *cap = nb;
     return true;
 }
@@ -255,8 +259,8 @@ bool prepare(drtopk_server* s, Slot& sl, Prep& pr) {
             else
                 pr.scratch = align_up(pr.ws_bytes, 256) + align_up(pr.plan_bytes, 256);
         }
-        if (!ensure_pinned(&sl.h_kth, &sl.h_kth_cap, 4 * rows) ||
-            (!pr.solo && !ensure_pinned(&sl.h_plan, &sl.h_plan_cap, pr.plan_bytes)))
+        if (!ensure_pinned(&sl.h_kth, &sl.h_kth_cap, 4 * rows, sl.retired) ||
+            (!pr.solo && !ensure_pinned(&sl.h_plan, &sl.h_plan_cap, pr.plan_bytes, sl.retired)))
             return false;
         bool ok = pr.scratch != ~0ull && s->arena.alloc(pr.out_bytes, &pr.out_off);
         if (ok && !s->arena.alloc(pr.scratch, &pr.scratch_off)) {
@@ -577,6 +581,7 @@ drtopk_status drtopk_server_destroy(drtopk_server* s) {
         if (sl.h_plan) (void)hipHostFree(sl.h_plan);
         if (sl.h_kth) (void)hipHostFree(sl.h_kth);
         if (sl.h_err) (void)hipHostFree(sl.h_err);
+        for (void* b : sl.retired) (void)hipHostFree(b);
     }
     if (s->d_arena) (void)hipFree(s->d_arena);
     delete s;
