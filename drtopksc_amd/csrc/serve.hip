@@ -454,7 +454,7 @@ void drtopk_server_config_default(drtopk_server_config* c) {
     std::memset(c, 0, sizeof(*c));
     c->max_inflight = 8;
     c->max_batch_rows = 4096;
-    c->solo_n = 1ull << 23;
+    c->solo_n = 1ull << 25;
     c->solo_k = 4096;
 }
 
@@ -469,7 +469,7 @@ drtopk_status drtopk_server_create(const drtopk_server_config* cfg, drtopk_serve
     if (!s->cfg.max_inflight) s->cfg.max_inflight = 8;
     if (s->cfg.max_inflight > 16) return fail(DRTOPK_ERR_INVALID_ARG, "max_inflight %u > 16", s->cfg.max_inflight);
     if (!s->cfg.max_batch_rows) s->cfg.max_batch_rows = 4096;
-    if (!s->cfg.solo_n) s->cfg.solo_n = 1ull << 23;
+    if (!s->cfg.solo_n) s->cfg.solo_n = 1ull << 25;
     if (!s->cfg.solo_k) s->cfg.solo_k = 4096;
     if (cfg->tuning) {
         s->tuning = *cfg->tuning;

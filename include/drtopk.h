@@ -273,7 +273,7 @@ typedef struct drtopk_server_config {
     uint64_t budget_bytes;    /* device arena for scratch and outputs (0: 64 GiB, capped at 90% of free memory) */
     uint32_t max_inflight;    /* launches on the device at once, one stream and worker thread each (0: 8) */
     uint32_t max_batch_rows;  /* rows per batched launch (0: 4096) */
-    uint64_t solo_n;          /* n at or above which a request runs as a single-vector call (0: 2^23) */
+    uint64_t solo_n;          /* n at or above which a request runs as a single-vector call (0: 2^25) */
     uint64_t solo_k;          /* k above which a request runs as a single-vector call (0: 4096) */
     const drtopk_tuning* tuning; /* batched launches' tuning table (NULL: built-in) */
     uint32_t reserved[4];     /* must be zero */

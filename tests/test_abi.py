@@ -217,7 +217,7 @@ def test_server_validation(L):
     h = ctypes.c_void_p()
     c = api.ServerConfig()
     L.drtopk_server_config_default(ctypes.byref(c))
-    assert (c.max_inflight, c.max_batch_rows, c.solo_n, c.solo_k) == (8, 4096, 1 << 23, 4096)
+    assert (c.max_inflight, c.max_batch_rows, c.solo_n, c.solo_k) == (8, 4096, 1 << 25, 4096)
     assert L.drtopk_server_create(ctypes.byref(c), ctypes.byref(h)) == 1  # no key store
     c.d_store = 4
     assert L.drtopk_server_create(ctypes.byref(c), ctypes.byref(h)) == 1  # misaligned
