@@ -695,6 +695,16 @@ void default_tuning(drtopk_tuning* t) {
     }
 }
 
+// The built-in table, built once.
+const drtopk_tuning& builtin_tuning() {
+    static const drtopk_tuning t = [] {
+        drtopk_tuning x;
+        default_tuning(&x);
+        return x;
+    }();
+    return t;
+}
+
 // Default choice for a cell the table leaves open: the reference's alpha policy
 // (baseline+filter+beta+shuffle/main.cu:83-89), 8-bit digits, 8192-key tiles.
 void choose(const drtopk_tuning& tu, u64 n, u64 k, int f32, int* alpha, int* nb, int* tile) {
@@ -727,21 +737,24 @@ drtopk_status drtopk_batch_plan(const drtopk_row* rows, uint64_t nrows, uint64_t
     if (!rows || !plan_bytes) return fail(DRTOPK_ERR_INVALID_ARG, "rows and plan_bytes must not be NULL");
     if (nrows == 0 || nrows > (1ull << 24)) return fail(DRTOPK_ERR_INVALID_ARG, "nrows %llu not in [1, 2^24]", (unsigned long long)nrows);
     if (keys_len >= (1ull << 48)) return fail(DRTOPK_ERR_UNSUPPORTED, "keys_len %llu >= 2^48", (unsigned long long)keys_len);
-    drtopk_tuning tu;
-    if (tuning) tu = *tuning;
-    else default_tuning(&tu);
-    for (int f = 0; f < 2; ++f)
-        for (int i = 0; i < 32; ++i)
-            for (int j = 0; j < 32; ++j) {
-                const u8 d = tu.digit_bits[f][i][j], tl = tu.tile_log2[f][i][j], al = tu.alpha[f][i][j];
-                if (d != DRTOPK_TUNE_DEFAULT && d != 4 && d != 8)
-                    return fail(DRTOPK_ERR_INVALID_ARG, "tuning digit_bits[%d][%d][%d] = %d not 4 or 8", f, i, j, d);
-                if (tl != DRTOPK_TUNE_DEFAULT && (tl < 10 || tl > MAX_TILE_LOG2))
-                    return fail(DRTOPK_ERR_INVALID_ARG, "tuning tile_log2[%d][%d][%d] = %d not in [10, 15]", f, i, j,
-                                tl);
-                if (al != DRTOPK_TUNE_DEFAULT && al > 30)
-                    return fail(DRTOPK_ERR_INVALID_ARG, "tuning alpha[%d][%d][%d] = %d not in [0, 30]", f, i, j, al);
-            }
+    // a caller's table is checked on every call; the built-in one is built once and needs no check
+    // (a server sizes every request with this call, so its fixed cost counts)
+    const drtopk_tuning& tu = tuning ? *tuning : builtin_tuning();
+    if (tuning)
+        for (int f = 0; f < 2; ++f)
+            for (int i = 0; i < 32; ++i)
+                for (int j = 0; j < 32; ++j) {
+                    const u8 d = tu.digit_bits[f][i][j], tl = tu.tile_log2[f][i][j], al = tu.alpha[f][i][j];
+                    if (d != DRTOPK_TUNE_DEFAULT && d != 4 && d != 8)
+                        return fail(DRTOPK_ERR_INVALID_ARG, "tuning digit_bits[%d][%d][%d] = %d not 4 or 8", f, i, j,
+                                    d);
+                    if (tl != DRTOPK_TUNE_DEFAULT && (tl < 10 || tl > MAX_TILE_LOG2))
+                        return fail(DRTOPK_ERR_INVALID_ARG, "tuning tile_log2[%d][%d][%d] = %d not in [10, 15]", f, i,
+                                    j, tl);
+                    if (al != DRTOPK_TUNE_DEFAULT && al > 30)
+                        return fail(DRTOPK_ERR_INVALID_ARG, "tuning alpha[%d][%d][%d] = %d not in [0, 30]", f, i, j,
+                                    al);
+                }
     std::vector<BatchRow> br(nrows);
     std::vector<u64> cost(nrows);
     u64 nch = 0, dl = 0, sl = 0;
